@@ -1,0 +1,258 @@
+"""Python host mirror of ConSGA's entry points over the C ABI.
+
+  Dataset.generate / Dataset.load   the drivers' input stage (src/ConSGA.cpp:154-211)
+  Engine.align_combo                align_combo (src/align.hpp:470-531): exact int32 scores
+  Engine.align_read_with_traceback  align_read_with_traceback (src/traceback.hpp:588-1031)
+  Engine.score_batch                batched align_combo over a read stream
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native as N
+
+
+class Dataset:
+    """String graph + char graph + contraction hierarchy held by libchsga_host."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        self._descs: dict[int, N.GraphDesc] = {}
+
+    @classmethod
+    def generate(cls, seed: int, backbone_bp: int, snp_rate: float, indel_rate: float, indel_len=(1, 10),
+                 sv_rate: float = 0.0, sv_len=(50, 5000)) -> "Dataset":
+        p = N.GraphParams(seed, backbone_bp, snp_rate, indel_rate, indel_len[0], indel_len[1], sv_rate,
+                          sv_len[0], sv_len[1])
+        h = C.c_void_p()
+        N._check_host(N.host_lib().chsga_host_generate(C.byref(p), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, char_adj: str, str_adj: str, ch: str | None, labels: str, offsets: str) -> "Dataset":
+        h = C.c_void_p()
+        enc = lambda s: None if s is None else str(s).encode()
+        N._check_host(N.host_lib().chsga_host_load(enc(char_adj), enc(str_adj), enc(ch), enc(labels), enc(offsets),
+                                                   C.byref(h)))
+        return cls(h.value)
+
+    def save(self, prefix: str) -> None:
+        N._check_host(N.host_lib().chsga_host_save(self._h, str(prefix).encode()))
+
+    def rebuild_ch(self, seed: int) -> None:
+        N._check_host(N.host_lib().chsga_host_build_ch(self._h, seed))
+        self._descs.clear()
+
+    def info(self) -> dict:
+        i = N.DatasetInfo()
+        N._check_host(N.host_lib().chsga_host_info(self._h, C.byref(i)))
+        return i.as_dict()
+
+    def desc(self, alphabet: int = N.ALPHABET_MISMATCH_N) -> N.GraphDesc:
+        if alphabet not in self._descs:
+            d = N.GraphDesc()
+            N._check_host(N.host_lib().chsga_host_describe(self._h, alphabet, C.byref(d)))
+            self._descs[alphabet] = d
+        return self._descs[alphabet]
+
+    # numpy views (valid while the dataset lives)
+    def labels(self) -> bytes:
+        d = self.desc()
+        return C.string_at(d.labels, d.V)
+
+    def arrays(self) -> dict:
+        d = self.desc()
+
+        def arr(ptr, n, dt):
+            if not ptr or n == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ptr, shape=(n,)).view(dt).copy()
+
+        return {
+            "V": d.V, "n_s": d.n_s,
+            "labels": np.frombuffer(self.labels(), np.uint8).copy(),
+            "in_off": arr(d.in_off, d.V + 1, np.uint64), "in_src": arr(d.in_src, d.E, np.uint32),
+            "str_off": arr(d.str_off, d.n_s + 1, np.uint64),
+            "str_out_off": arr(d.str_out_off, d.n_s + 1, np.uint64), "str_out_dst": arr(d.str_out_dst, d.m_str, np.uint32),
+            "layer_off": arr(d.layer_off, d.num_layers + 1, np.uint64), "cc_off": arr(d.cc_off, d.num_cc + 1, np.uint64),
+            "rank": arr(d.rank, d.n_s, np.uint32),
+            "up_off": arr(d.up_off, d.n_s + 1, np.uint64), "up_v": arr(d.up_v, d.m_up, np.uint32),
+            "up_w": arr(d.up_w, d.m_up, np.int32), "dn_off": arr(d.dn_off, d.n_s + 1, np.uint64),
+            "dn_v": arr(d.dn_v, d.m_dn, np.uint32), "dn_w": arr(d.dn_w, d.m_dn, np.int32),
+            "up_mid": arr(d.up_mid, d.m_up, np.int32), "dn_mid": arr(d.dn_mid, d.m_dn, np.int32),
+        }
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            try:
+                N.host_lib().chsga_host_free(self._h)
+            except Exception:
+                pass
+            self._h = C.c_void_p()
+
+
+class ReadSet:
+    """Reads as one concatenated byte string plus offsets."""
+
+    def __init__(self, data: bytes, offsets: np.ndarray):
+        self.data = data
+        self.offsets = np.asarray(offsets, dtype=np.int64)
+
+    @classmethod
+    def _from_handle(cls, h: C.c_void_p) -> "ReadSet":
+        lib = N.host_lib()
+        n = lib.chsga_reads_count(h)
+        off = np.ctypeslib.as_array(C.cast(lib.chsga_reads_offsets(h), N.i64p), shape=(n + 1,)).copy()
+        data = C.string_at(lib.chsga_reads_data(h), int(off[-1]))
+        lib.chsga_reads_free(h)
+        return cls(data, off)
+
+    @classmethod
+    def generate(cls, ds: Dataset, seed: int, n_reads: int, length: float, len_mode: int = 0, len_sigma: float = 0.0,
+                 len_min: int = 1, len_max: int = 1 << 40, sub: float = 0.0, ins: float = 0.0, dele: float = 0.0,
+                 n_rate: float = 0.0, mix_frac: float = 0.0, len2=(1, 1), err_scale=(1.0, 1.0)) -> "ReadSet":
+        p = N.ReadParams(seed, n_reads, len_mode, float(length), float(len_sigma), len_min, len_max, sub, ins, dele,
+                         n_rate, mix_frac, len2[0], len2[1], err_scale[0], err_scale[1])
+        h = C.c_void_p()
+        N._check_host(N.host_lib().chsga_host_gen_reads(ds._h, C.byref(p), C.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def load(cls, path: str, cap: int = 0) -> "ReadSet":
+        h = C.c_void_p()
+        N._check_host(N.host_lib().chsga_host_load_reads(str(path).encode(), cap, C.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def from_list(cls, reads: list) -> "ReadSet":
+        bs = [r.encode() if isinstance(r, str) else bytes(r) for r in reads]
+        off = np.zeros(len(bs) + 1, np.int64)
+        off[1:] = np.cumsum([len(b) for b in bs])
+        return cls(b"".join(bs), off)
+
+    def __len__(self) -> int:
+        return len(self.offsets) - 1
+
+    def __getitem__(self, i: int) -> bytes:
+        return self.data[self.offsets[i]:self.offsets[i + 1]]
+
+    def lengths(self) -> np.ndarray:
+        return np.diff(self.offsets)
+
+    def subset(self, idx) -> "ReadSet":
+        return ReadSet.from_list([self[int(i)] for i in idx])
+
+    def ptrs(self):
+        """(char** array, int64 lengths) views for the C ABI; keep the result alive."""
+        buf = C.create_string_buffer(self.data, len(self.data) + 1)
+        base = C.addressof(buf)
+        n = len(self)
+        arr = (C.c_char_p * max(n, 1))()
+        for i in range(n):
+            arr[i] = C.cast(base + int(self.offsets[i]), C.c_char_p)
+        lens = np.ascontiguousarray(self.lengths(), dtype=np.int64)
+        return buf, arr, lens
+
+
+@dataclass
+class TracebackResult:
+    score: int
+    end_vertex: int
+    ops: np.ndarray  # structured: row int64, vertex uint32, op uint8
+
+    def cigar(self) -> str:
+        """ops_to_cigar (src/traceback.hpp:1081-1110): D / M / I run lengths."""
+        if len(self.ops) == 0:
+            return ""
+        ch = {N.OP_DELETE: "D", N.OP_MATCHSUB: "M"}
+        out, prev, run = [], None, 0
+        for o in self.ops["op"]:
+            c = ch.get(int(o), "I")
+            if c == prev:
+                run += 1
+            else:
+                if prev is not None:
+                    out.append(f"{run}{prev}")
+                prev, run = c, 1
+        out.append(f"{run}{prev}")
+        return "".join(out)
+
+
+OP_DTYPE = np.dtype([("row", np.int64), ("vertex", np.uint32), ("op", np.uint8)], align=True)
+
+
+class Engine:
+    """A chsga_ctx on one GPU."""
+
+    def __init__(self, ds: Dataset, alphabet: int = N.ALPHABET_MISMATCH_N, device: int = 0):
+        self.ds = ds
+        self.alphabet = alphabet
+        self._lib = N.engine_lib()
+        h = C.c_void_p()
+        N._check_engine(self._lib.chsga_ctx_create(C.byref(ds.desc(alphabet)), device, C.byref(h)))
+        self._h = h
+        self.last_stats: dict = {}
+
+    def set_option(self, opt: int, value: int) -> None:
+        N._check_engine(self._lib.chsga_ctx_set_option(self._h, opt, value))
+
+    def align_combo(self, reads: ReadSet, per_read_times: bool = False):
+        n = len(reads)
+        buf, arr, lens = reads.ptrs()
+        scores = np.zeros(n, np.int32)
+        st = N.Stats()
+        tt = np.zeros(n, np.float64) if per_read_times else None
+        ts = np.zeros(n, np.float64) if per_read_times else None
+        ti = np.zeros(n, np.float64) if per_read_times else None
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+        N._check_engine(self._lib.chsga_score_reads(self._h, arr, lens.ctypes.data_as(N.i64p), n,
+                                                    scores.ctypes.data_as(N.i32p), dp(tt), dp(ts), dp(ti),
+                                                    C.byref(st)))
+        self.last_stats = st.as_dict()
+        if per_read_times:
+            return scores, (tt, ts, ti)
+        return scores
+
+    def last_row(self, read: bytes) -> np.ndarray:
+        V = self.ds.desc(self.alphabet).V
+        row = np.zeros(V, np.int32)
+        N._check_engine(self._lib.chsga_last_row(self._h, read, len(read), row.ctypes.data_as(N.i32p)))
+        return row
+
+    def score_batch(self, reads: ReadSet, slots: int = 64, order=None, use_graphs: bool = False) -> np.ndarray:
+        n = len(reads)
+        buf, arr, lens = reads.ptrs()
+        scores = np.zeros(n, np.int32)
+        ordv = None
+        if order is not None:
+            ordv = np.ascontiguousarray(order, dtype=np.int64)
+        opts = N.BatchOpts(slots, 1 if use_graphs else 0, ordv.ctypes.data_as(N.i64p) if ordv is not None else None)
+        st = N.Stats()
+        N._check_engine(self._lib.chsga_score_batch(self._h, arr, lens.ctypes.data_as(N.i64p), n, C.byref(opts),
+                                                    scores.ctypes.data_as(N.i32p), C.byref(st)))
+        self.last_stats = st.as_dict()
+        return scores
+
+    def align_read_with_traceback(self, read: bytes, k: int = 500) -> TracebackResult:
+        p = C.POINTER(N.Path_)()
+        st = N.Stats()
+        N._check_engine(self._lib.chsga_traceback_read(self._h, read, len(read), k, C.byref(p), C.byref(st)))
+        self.last_stats = st.as_dict()
+        try:
+            n = p.contents.n_ops
+            raw = C.string_at(p.contents.ops, n * C.sizeof(N.AlignOp))
+            ops = np.frombuffer(raw, dtype=OP_DTYPE).copy()
+            return TracebackResult(int(p.contents.score), int(p.contents.end_vertex), ops)
+        finally:
+            self._lib.chsga_path_free(p)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.chsga_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()

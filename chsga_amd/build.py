@@ -1,0 +1,95 @@
+"""Build recipe for the native parts (no cmake/ninja needed).
+
+  chsga_amd/lib/libchsga_host.so   host formats, generator, CH builder (g++)
+  chsga_amd/lib/libchsga.so        gfx950 HIP engine behind include/chsga/abi.h (hipcc)
+  chsga_amd/bin/*                  drop-in drivers (ConSGA_*, ConSGA_traceback_*, build_pch, chsga_gen)
+  oracle/build/liboracle.so        CPU restatement used as the checker (g++, OpenMP)
+
+Run `python -m chsga_amd.build` (or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "chsga_amd"
+CSRC = PKG / "csrc"
+LIB = PKG / "lib"
+BIN = PKG / "bin"
+INC = ROOT / "include"
+ORACLE = ROOT / "oracle"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
+
+
+def _run(cmds: list[list[str]], verbose: bool) -> None:
+    procs = []
+    for c in cmds:
+        if verbose:
+            print(" ".join(str(x) for x in c), flush=True)
+        procs.append((c, subprocess.Popen([str(x) for x in c], stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    failed = []
+    for c, p in procs:
+        out, _ = p.communicate()
+        text = out.decode(errors="replace")
+        if p.returncode != 0:
+            failed.append((c, text))
+        elif verbose and text.strip():
+            print(text)
+    if failed:
+        msg = "\n\n".join(" ".join(str(x) for x in c) + "\n" + t for c, t in failed)
+        raise RuntimeError("native build failed:\n" + msg)
+
+
+def build(verbose: bool = False) -> None:
+    LIB.mkdir(exist_ok=True)
+    BIN.mkdir(exist_ok=True)
+    (ORACLE / "build").mkdir(exist_ok=True)
+    host_src = [CSRC / "host" / "host.cpp", CSRC / "host" / "ch_build.cpp"]
+    hip_src = CSRC / "hip" / "engine.hip"
+    stage1 = [
+        [CXX, *CXX_FLAGS, "-fopenmp", "-shared", "-I", INC, *host_src, "-lz",
+         "-o", LIB / "libchsga_host.so"],
+        [HIPCC, *HIP_FLAGS, "-shared", "-I", INC, "-I", CSRC / "hip", hip_src, "-lpthread",
+         "-o", LIB / "libchsga.so"],
+        [CXX, *CXX_FLAGS, "-fopenmp", "-shared", "-I", INC, ORACLE / "oracle.cpp",
+         "-o", ORACLE / "build" / "liboracle.so"],
+    ]
+    _run(stage1, verbose)
+    rpath = "-Wl,-rpath,$ORIGIN/../lib"
+    drv = CSRC / "drivers"
+    stage2 = []
+    for name, src, defs in [
+        ("ConSGA_mismatchN_dag", "consga.cpp", ["-DCHSGA_ALPHABET=5"]),
+        ("ConSGA_matchN_dag", "consga.cpp", ["-DCHSGA_ALPHABET=4"]),
+        ("ConSGA_traceback_mismatchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=5"]),
+        ("ConSGA_traceback_matchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=4"]),
+        ("build_pch", "build_pch.cpp", []),
+        ("chsga_gen", "chsga_gen.cpp", []),
+    ]:
+        if (drv / src).exists():
+            stage2.append([CXX, *CXX_FLAGS, *defs, "-I", INC, drv / src, "-L", LIB, "-lchsga", "-lchsga_host",
+                           rpath, "-o", BIN / name])
+    if stage2:
+        _run(stage2, verbose)
+
+
+def built() -> bool:
+    return (LIB / "libchsga.so").exists() and (LIB / "libchsga_host.so").exists()
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print("built:", ", ".join(sorted(p.name for p in LIB.glob("*.so"))))

@@ -1,0 +1,476 @@
+// MI355X engine behind include/chsga/abi.h: context set-up, the plain
+// one-read-per-launch-sequence path, the batched path and traceback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "chsga/abi.h"
+#include "engine_internal.hpp"
+
+namespace chsga_engine {
+
+thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevMem::alloc(size_t bytes) {
+  release();
+  if (bytes == 0) return;
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  n = bytes;
+}
+void DevMem::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+
+template <class T>
+static void upload(DevMem& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(std::max<size_t>(1, h.size()) * sizeof(T));
+  if (!h.empty()) hip_check(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+static int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace chsga_engine
+
+using namespace chsga_engine;
+using namespace chsga_dev;
+
+// ---------------------------------------------------------------------------
+// context set-up
+// ---------------------------------------------------------------------------
+static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
+  if (!g || g->V <= 0 || !g->labels || !g->in_off || !g->in_src || g->n_s <= 0 || !g->str_off || !g->rank ||
+      !g->layer_off || !g->cc_off || !g->up_off || !g->dn_off)
+    throw std::invalid_argument("incomplete graph descriptor");
+  if (g->alphabet != CHSGA_ALPHABET_MATCH_N && g->alphabet != CHSGA_ALPHABET_MISMATCH_N)
+    throw std::invalid_argument("alphabet must be 4 (matchN) or 5 (mismatchN)");
+  if (g->match_score < 0 || g->subst_score < 0 || g->del_score < 0 || g->ins_score < 0)
+    throw std::invalid_argument("edit costs must be non-negative");
+  if (g->V >= (int64_t(1) << 31) || g->E >= (int64_t(1) << 32) || g->n_s >= (int64_t(1) << 31))
+    throw std::invalid_argument("graph too large for 32-bit indices");
+  if (int64_t(g->str_off[g->n_s]) != g->V || g->str_off[0] != 0)
+    throw std::invalid_argument("label offsets do not cover the char vertices");
+  if (int64_t(g->in_off[g->V]) != g->E) throw std::invalid_argument("char in-CSR size mismatch");
+  c->V = g->V;
+  c->E = g->E;
+  c->n_s = g->n_s;
+  c->alphabet = g->alphabet;
+  c->msc = g->match_score;
+  c->ssc = g->subst_score;
+  c->dsc = g->del_score;
+  c->isc = g->ins_score;
+  const int64_t V = c->V, n_s = c->n_s;
+
+  // host copies used by traceback
+  c->h_labels.assign(g->labels, g->labels + V);
+  c->h_str_off.resize(n_s + 1);
+  for (int64_t s = 0; s <= n_s; ++s) c->h_str_off[s] = uint32_t(g->str_off[s]);
+  c->h_in_off.resize(V + 1);
+  for (int64_t v = 0; v <= V; ++v) c->h_in_off[v] = uint32_t(g->in_off[v]);
+  c->h_in_src.assign(g->in_src, g->in_src + c->E);
+  if (g->str_out_off && g->str_out_dst) {
+    c->h_sin_off.assign(n_s + 1, 0);
+    c->h_sin_src.resize(g->m_str);
+    for (int64_t u = 0; u < n_s; ++u)
+      for (uint64_t e = g->str_out_off[u]; e < g->str_out_off[u + 1]; ++e) ++c->h_sin_off[g->str_out_dst[e] + 1];
+    for (int64_t s = 0; s < n_s; ++s) c->h_sin_off[s + 1] += c->h_sin_off[s];
+    std::vector<uint32_t> cur(c->h_sin_off.begin(), c->h_sin_off.end() - 1);
+    for (int64_t u = 0; u < n_s; ++u)
+      for (uint64_t e = g->str_out_off[u]; e < g->str_out_off[u + 1]; ++e)
+        c->h_sin_src[cur[g->str_out_dst[e]]++] = uint32_t(u);
+  }
+
+  std::vector<uint32_t> first_bits((V + 32) / 32, 0);
+  for (int64_t s = 0; s < n_s; ++s) first_bits[c->h_str_off[s] >> 5] |= 1u << (c->h_str_off[s] & 31);
+
+  // tiles of whole string vertices
+  const int64_t T = c->tile_chars;
+  std::vector<uint32_t> tile_char{0}, tile_str{0};
+  int64_t acc = 0;
+  for (int64_t s = 0; s < n_s; ++s) {
+    const int64_t len = c->h_str_off[s + 1] - c->h_str_off[s];
+    if (acc > 0 && acc + len > T) {
+      tile_char.push_back(c->h_str_off[s]);
+      tile_str.push_back(uint32_t(s));
+      acc = 0;
+    }
+    acc += len;
+  }
+  tile_char.push_back(uint32_t(V));
+  tile_str.push_back(uint32_t(n_s));
+  c->n_tiles = int64_t(tile_char.size()) - 1;
+
+  // CH sweep in string-id space.  step(c) = layer index of c inside its
+  // component; component layers are processed side by side, which is exact
+  // because components share no edges.
+  std::vector<uint32_t> ch_to_og(n_s);
+  for (int64_t i = 0; i < n_s; ++i) ch_to_og[g->rank[i]] = uint32_t(i);
+  std::vector<int32_t> step(n_s, -1);
+  int32_t J = 0;
+  for (int64_t cc = 0; cc < g->num_cc; ++cc)
+    for (uint64_t l = g->cc_off[cc]; l < g->cc_off[cc + 1]; ++l) {
+      const int32_t j = int32_t(l - g->cc_off[cc]);
+      J = std::max(J, j + 1);
+      for (uint64_t x = g->layer_off[l]; x < g->layer_off[l + 1]; ++x) step[x] = j;
+    }
+  for (int64_t x = 0; x < n_s; ++x)
+    if (step[x] < 0) throw std::invalid_argument("CH layers do not cover every vertex");
+  std::vector<uint32_t> order(n_s);  // CH ids in sweep order
+  std::iota(order.begin(), order.end(), 0u);
+  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    if (step[a] != step[b]) return step[a] < step[b];
+    return ch_to_og[a] < ch_to_og[b];
+  });
+  std::vector<uint32_t> kpos(n_s);
+  for (int64_t k = 0; k < n_s; ++k) kpos[order[k]] = uint32_t(k);
+  c->h_lvl_off.assign(J + 1, 0);
+  for (int64_t x = 0; x < n_s; ++x) ++c->h_lvl_off[step[x] + 1];
+  for (int32_t j = 0; j < J; ++j) c->h_lvl_off[j + 1] += c->h_lvl_off[j];
+  c->J = J;
+  std::vector<uint32_t> sw_vert(n_s);
+  for (int64_t k = 0; k < n_s; ++k) sw_vert[k] = ch_to_og[order[k]];
+  auto code = [&](uint32_t og) -> int32_t {
+    const bool multi = (c->h_str_off[og + 1] - c->h_str_off[og]) > 1;
+    return int32_t(og | (multi ? 0x80000000u : 0u));
+  };
+  // up pulls: transpose of the up-edges (x -> y stored at x)
+  std::vector<uint32_t> up_off(n_s + 1, 0);
+  for (int64_t x = 0; x < n_s; ++x)
+    for (uint64_t e = g->up_off[x]; e < g->up_off[x + 1]; ++e) {
+      const uint32_t y = g->up_v[e];
+      if (step[y] <= step[x]) throw std::invalid_argument("CH up-edge does not go to a higher layer");
+      ++up_off[kpos[y] + 1];
+    }
+  for (int64_t k = 0; k < n_s; ++k) up_off[k + 1] += up_off[k];
+  std::vector<int2> up_in(up_off[n_s]);
+  {
+    std::vector<uint32_t> cur(up_off.begin(), up_off.end() - 1);
+    for (int64_t x = 0; x < n_s; ++x)
+      for (uint64_t e = g->up_off[x]; e < g->up_off[x + 1]; ++e) {
+        const uint32_t y = g->up_v[e];
+        up_in[cur[kpos[y]]++] = make_int2(code(ch_to_og[x]), g->up_w[e]);
+      }
+  }
+  std::vector<uint32_t> dn_off(n_s + 1, 0);
+  std::vector<int2> dn_in;
+  dn_in.reserve(g->m_dn);
+  for (int64_t k = 0; k < n_s; ++k) {
+    const uint32_t x = order[k];
+    for (uint64_t e = g->dn_off[x]; e < g->dn_off[x + 1]; ++e) {
+      const uint32_t y = g->dn_v[e];
+      if (step[y] <= step[x]) throw std::invalid_argument("CH down-edge does not come from a higher layer");
+      dn_in.push_back(make_int2(code(ch_to_og[y]), g->dn_w[e]));
+    }
+    dn_off[k + 1] = uint32_t(dn_in.size());
+  }
+  // first step handled by the one-workgroup top kernel
+  int32_t jt = J;
+  if (c->top_level_opt >= 0) {
+    jt = std::min<int32_t>(int32_t(c->top_level_opt), J);
+  } else {
+    while (jt > 0 && (c->h_lvl_off[jt] - c->h_lvl_off[jt - 1]) <= 2048 && (n_s - c->h_lvl_off[jt - 1]) <= 16384)
+      --jt;
+  }
+  c->j_top = jt;
+
+  hipStream_t s = c->stream;
+  std::vector<uint8_t> labels(g->labels, g->labels + V);
+  upload(c->d_labels, labels, s);
+  upload(c->d_in_off, c->h_in_off, s);
+  upload(c->d_in_src, c->h_in_src, s);
+  upload(c->d_first_bits, first_bits, s);
+  upload(c->d_tile_char, tile_char, s);
+  upload(c->d_tile_str, tile_str, s);
+  upload(c->d_str_first, c->h_str_off, s);
+  upload(c->d_sw_vert, sw_vert, s);
+  upload(c->d_lvl_off, c->h_lvl_off, s);
+  upload(c->d_up_off, up_off, s);
+  upload(c->d_up_in, up_in, s);
+  upload(c->d_dn_off, dn_off, s);
+  upload(c->d_dn_in, dn_in, s);
+  c->h_up_off = std::move(up_off);
+  c->h_dn_off = std::move(dn_off);
+  c->d_F.alloc(sizeof(int32_t) * n_s);
+  c->d_Lst.alloc(sizeof(int32_t) * n_s);
+  c->d_row[0].alloc(sizeof(int32_t) * V);
+  c->d_row[1].alloc(sizeof(int32_t) * V);
+  hip_check(hipStreamSynchronize(s), "context upload");
+}
+
+void chsga_ctx::launch_row(const int32_t* H, int32_t* D, uint8_t rc) {
+  const hipStream_t s = stream;
+  if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
+  hipLaunchKernelGGL(k_row_tile, dim3(unsigned(n_tiles)), dim3(CHSGA_BLOCK), 0, s, H, D, (int32_t*)d_F.p,
+                     (int32_t*)d_Lst.p, (const uint8_t*)d_labels.p, (const uint32_t*)d_in_off.p,
+                     (const uint32_t*)d_in_src.p, (const uint32_t*)d_first_bits.p, (const uint32_t*)d_tile_char.p,
+                     (const uint32_t*)d_tile_str.p, V, rc, alphabet, msc, ssc, dsc, isc);
+  if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
+  launches += 1;
+  launch_insertion(D);
+  if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
+}
+
+void chsga_ctx::launch_insertion(int32_t* D) {
+  const hipStream_t s = stream;
+  int32_t* F = (int32_t*)d_F.p;
+  const int32_t* L = (const int32_t*)d_Lst.p;
+  for (int32_t j = 1; j < j_top; ++j) {
+    const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
+    if (h_up_off[k1] == h_up_off[k0]) continue;
+    hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F, L,
+                       (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p, k0, k1,
+                       isc);
+    launches += 1;
+  }
+  if (j_top < J) {
+    hipLaunchKernelGGL(k_sweep_top, dim3(1), dim3(1024), 0, s, F, L, (const uint32_t*)d_sw_vert.p,
+                       (const uint32_t*)d_lvl_off.p, j_top, J, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p,
+                       (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, isc);
+    launches += 1;
+  }
+  for (int32_t j = std::min(j_top, J) - 1; j >= 0; --j) {
+    const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
+    if (h_dn_off[k1] == h_dn_off[k0]) continue;
+    hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F, L,
+                       (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, k0, k1,
+                       isc);
+    launches += 1;
+  }
+  hipLaunchKernelGGL(k_post, dim3(unsigned((n_s + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0, s, D,
+                     (const int32_t*)F, (const uint32_t*)d_str_first.p, n_s, isc);
+  launches += 1;
+}
+
+void chsga_ctx::launch_init(int32_t* H, uint8_t c0) {
+  hipLaunchKernelGGL(k_init_row, dim3(unsigned((V + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0, stream,
+                     H, (const uint8_t*)d_labels.p, V, c0, alphabet, msc, ssc);
+  launches += 1;
+}
+
+// Forward pass of one read into the two-row ping-pong buffer; returns the
+// buffer index holding the last row.  ckpt (optional) receives row 0 and
+// every row i with i % k == 0.
+int chsga_ctx::forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows) {
+  int32_t* R[2] = {(int32_t*)d_row[0].p, (int32_t*)d_row[1].p};
+  int cur = 0;
+  launch_init(R[cur], uint8_t(read[0]));
+  if (ckpt_rows)
+    hip_check(hipMemcpyAsync(ckpt_rows[0], R[cur], sizeof(int32_t) * V, hipMemcpyDeviceToDevice, stream), "ckpt");
+  for (int64_t i = 1; i < len; ++i) {
+    if (profile && ev_used + 3 > ev_pool.size()) flush_profile();
+    launch_row(R[cur], R[cur ^ 1], uint8_t(read[i]));
+    cur ^= 1;
+    if (ckpt_rows && i % k == 0)
+      hip_check(hipMemcpyAsync(ckpt_rows[i / k], R[cur], sizeof(int32_t) * V, hipMemcpyDeviceToDevice, stream),
+                "ckpt");
+  }
+  rows += len;
+  return cur;
+}
+
+void chsga_ctx::flush_profile() {
+  if (!profile || ev_used == 0) return;
+  hip_check(hipEventSynchronize(ev_pool[ev_used - 1]), "event sync");
+  for (size_t i = 0; i + 2 < ev_used; i += 3) {
+    float a = 0, b = 0;
+    hip_check(hipEventElapsedTime(&a, ev_pool[i], ev_pool[i + 1]), "elapsed");
+    hip_check(hipEventElapsedTime(&b, ev_pool[i + 1], ev_pool[i + 2]), "elapsed");
+    prof_row_ms += a;
+    prof_ins_ms += b;
+    prof_rows += 1;
+  }
+  ev_used = 0;
+}
+
+chsga_ctx::~chsga_ctx() {
+  for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+#define ENGINE_GUARD(...)                  \
+  try {                                     \
+    __VA_ARGS__;                                   \
+    return CHSGA_OK;                        \
+  } catch (const std::invalid_argument& e) { \
+    set_error(e.what());                    \
+    return CHSGA_EINVAL;                    \
+  } catch (const HipError& e) {             \
+    set_error(e.what());                    \
+    return CHSGA_EDEVICE;                   \
+  } catch (const std::bad_alloc&) {         \
+    set_error("host allocation failed");    \
+    return CHSGA_ENOMEM;                    \
+  } catch (const std::exception& e) {       \
+    set_error(e.what());                    \
+    return CHSGA_EINTERNAL;                 \
+  }
+
+extern "C" {
+
+const char* chsga_last_error(void) { return g_err.c_str(); }
+
+int chsga_device_count(int* count) {
+  if (!count) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  ENGINE_GUARD({
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    *count = n;
+  })
+}
+
+int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out) {
+  if (!out) {
+    set_error("null output pointer");
+    return CHSGA_EINVAL;
+  }
+  *out = nullptr;
+  chsga_ctx* c = nullptr;
+  ENGINE_GUARD({
+    c = new chsga_ctx();
+    try {
+      c->device = device;
+      hip_check(hipSetDevice(device), "hipSetDevice");
+      hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+      build_context(c, g);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  })
+}
+
+void chsga_ctx_destroy(chsga_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->worker.joinable()) ctx->worker.join();
+  (void)hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
+  if (!ctx) {
+    set_error("null context");
+    return CHSGA_EINVAL;
+  }
+  ENGINE_GUARD({
+    switch (option) {
+      case CHSGA_OPT_PROFILE:
+        ctx->profile = value != 0;
+        if (ctx->profile && ctx->ev_pool.empty()) {
+          ctx->ev_pool.resize(3 * 512);
+          for (auto& e : ctx->ev_pool) hip_check(hipEventCreate(&e), "hipEventCreate");
+        }
+        break;
+      case CHSGA_OPT_USE_GRAPHS:
+        ctx->use_graphs = value != 0;
+        break;
+      case CHSGA_OPT_TOP_LEVEL:
+        ctx->top_level_opt = value;
+        if (value >= 0) ctx->j_top = int32_t(std::min<int64_t>(value, ctx->J));
+        break;
+      case CHSGA_OPT_TILE_CHARS:
+        throw std::invalid_argument("tile size is fixed at context creation");
+      default:
+        throw std::invalid_argument("unknown option");
+    }
+  })
+}
+
+int chsga_score_reads(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n, int32_t* scores_out,
+                      double* total_sec, double* sub_sec, double* ins_sec, chsga_stats* stats) {
+  if (!ctx || (n > 0 && (!reads || !lens || !scores_out))) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  ENGINE_GUARD({
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    for (int64_t r = 0; r < n; ++r)
+      if (lens[r] <= 0 || !reads[r]) throw std::invalid_argument("empty read");
+    const int64_t t0 = now_ns();
+    ctx->launches = 0;
+    ctx->rows = 0;
+    ctx->prof_row_ms = ctx->prof_ins_ms = 0;
+    ctx->prof_rows = 0;
+    ctx->ev_used = 0;
+    DevMem keys;
+    keys.alloc(sizeof(unsigned long long) * std::max<int64_t>(1, n));
+    unsigned long long* dk = (unsigned long long*)keys.p;
+    hipLaunchKernelGGL(k_fill_u64, dim3(unsigned((n + 255) / 256 + 1)), dim3(256), 0, ctx->stream, dk, n, ~0ull);
+    int64_t cells = 0;
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t r0 = now_ns();
+      const double pr0 = ctx->prof_row_ms, pi0 = ctx->prof_ins_ms;
+      const int cur = ctx->forward(reads[r], lens[r], 0, nullptr);
+      hipLaunchKernelGGL(k_argmin, dim3(512), dim3(CHSGA_BLOCK), 0, ctx->stream, (const int32_t*)ctx->d_row[cur].p,
+                         ctx->V, dk + r);
+      cells += lens[r] * ctx->V;
+      if (total_sec || sub_sec || ins_sec || ctx->profile) {
+        ctx->flush_profile();
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        const double sec = double(now_ns() - r0) * 1e-9;
+        if (total_sec) total_sec[r] = sec;
+        if (sub_sec) sub_sec[r] = ctx->profile ? (ctx->prof_row_ms - pr0) * 1e-3 : 0.0;
+        if (ins_sec) ins_sec[r] = ctx->profile ? (ctx->prof_ins_ms - pi0) * 1e-3 : 0.0;
+      }
+    }
+    std::vector<unsigned long long> hk(n);
+    if (n)
+      hip_check(hipMemcpyAsync(hk.data(), dk, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, ctx->stream),
+                "copy scores");
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    hip_check(hipGetLastError(), "kernel launch");
+    for (int64_t r = 0; r < n; ++r) scores_out[r] = int32_t(hk[r] >> 32);
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->wall_ms = double(now_ns() - t0) * 1e-6;
+      stats->row_kernel_ms = ctx->prof_row_ms;
+      stats->row_kernel_launches = ctx->prof_rows;
+      stats->sweep_ms = ctx->prof_ins_ms;
+      stats->cells = cells;
+      stats->kernel_launches = ctx->launches;
+      stats->rows = ctx->rows;
+    }
+  })
+}
+
+int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_out) {
+  if (!ctx || !read || len <= 0 || !row_out) {
+    set_error("bad argument");
+    return CHSGA_EINVAL;
+  }
+  ENGINE_GUARD({
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const bool prof = ctx->profile;
+    ctx->profile = false;
+    const int cur = ctx->forward(read, len, 0, nullptr);
+    ctx->profile = prof;
+    hip_check(hipMemcpyAsync(row_out, ctx->d_row[cur].p, sizeof(int32_t) * ctx->V, hipMemcpyDeviceToHost,
+                             ctx->stream),
+              "copy row");
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+  })
+}
+
+}  // extern "C"
+
+#include "batch.hip.inc"
+#include "traceback.hip.inc"

@@ -1,0 +1,72 @@
+// Context of the MI355X engine (private to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "chsga/abi.h"
+#include "kernels.hip.inc"
+
+namespace chsga_engine {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+void set_error(const std::string& m);
+void hip_check(hipError_t e, const char* what);
+
+struct DevMem {
+  void* p = nullptr;
+  size_t n = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() { release(); }
+  void alloc(size_t bytes);
+  void release();
+};
+
+}  // namespace chsga_engine
+
+struct chsga_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // graph
+  int64_t V = 0, E = 0, n_s = 0;
+  int32_t alphabet = 5, msc = 0, ssc = 1, dsc = 1, isc = 1;
+  int64_t n_tiles = 0;
+  int32_t J = 0, j_top = 0;
+  int64_t top_level_opt = -1;
+  int64_t tile_chars = 2048;
+  std::vector<uint32_t> h_lvl_off, h_up_off, h_dn_off;
+  // host copies for traceback
+  std::string h_labels;
+  std::vector<uint32_t> h_str_off, h_in_off, h_in_src, h_sin_off, h_sin_src;
+  // device graph
+  chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
+  chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
+  // working rows
+  chsga_engine::DevMem d_F, d_Lst, d_row[2];
+  // options and counters
+  bool profile = false, use_graphs = false;
+  int64_t launches = 0, rows = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double prof_row_ms = 0, prof_ins_ms = 0;
+  int64_t prof_rows = 0;
+  // asynchronous batch
+  std::thread worker;
+  int worker_status = 0;
+  chsga_stats worker_stats{};
+
+  ~chsga_ctx();
+  void launch_init(int32_t* H, uint8_t c0);
+  void launch_row(const int32_t* H, int32_t* D, uint8_t rc);
+  void launch_insertion(int32_t* D);
+  int forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows);
+  void flush_profile();
+};

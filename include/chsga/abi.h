@@ -1,0 +1,190 @@
+/*
+ * chsga/abi.h -- C ABI of the MI355X (gfx950) ConSGA graph-DP engine.
+ *
+ * The reference exposes this hot path only as C++ templates called by its
+ * drivers; every entry point below replaces one of them:
+ *
+ *   chsga_ctx_create      <- the graph set-up the drivers do before aligning:
+ *                            build_in_csr (src/align.hpp:220-242), rank
+ *                            inversion contracted_str_to_og[rank[i]] = i
+ *                            (src/ConSGA.cpp:203-206), VertexLabelMapper
+ *                            (src/align.hpp:177-202), PchQuery (ConSGA.cpp:197)
+ *   chsga_score_reads     <- align_combo (src/align.hpp:470-531), which runs
+ *                            align_read_combo (:399-468) once per read
+ *   chsga_traceback_read  <- align_read_with_traceback (src/traceback.hpp:588-1031)
+ *   chsga_score_batch     <- align_combo over a read stream, batched (no
+ *                            counterpart: the reference aligns reads one by one,
+ *                            src/align.hpp:488)
+ *   chsga_ctx_destroy     <- the delete[] of the row buffers (src/align.hpp:526)
+ *
+ * Rules of the boundary:
+ *  - plain pointers and sizes only; no exceptions cross it; every function
+ *    returns CHSGA_OK (0) or a negative status, with chsga_last_error()
+ *    giving a message for the calling thread;
+ *  - the library copies everything chsga_ctx_create is given into device
+ *    memory; the caller may free its arrays afterwards;
+ *  - the library prints nothing;
+ *  - one context per host thread; calls on one context are serialised on its
+ *    own HIP stream and are not re-entrant.
+ *
+ * Scores are exact int32 edit costs (match/subst/del/ins), identical to the
+ * reference's LogScore=int arithmetic (src/ConSGA.cpp:216).
+ */
+#ifndef CHSGA_ABI_H
+#define CHSGA_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHSGA_OK 0
+#define CHSGA_EINVAL (-1)   /* bad argument or inconsistent graph */
+#define CHSGA_ENOMEM (-2)   /* device or host allocation failed */
+#define CHSGA_EDEVICE (-3)  /* HIP runtime error */
+#define CHSGA_EIO (-4)      /* file or format error */
+#define CHSGA_EINTERNAL (-5)
+
+/* Alphabets, after src/config.hpp:3-15 and src/ParSGAoverrides/csr_align.hpp:41-50.
+ * MISMATCH_N (5 symbols): sigma(a,b) = a==b ? match : subst   (N == N matches)
+ * MATCH_N    (4 symbols): sigma(a,b) = (a=='N'||b=='N'||a==b) ? match : subst */
+#define CHSGA_ALPHABET_MATCH_N 4
+#define CHSGA_ALPHABET_MISMATCH_N 5
+
+/* Back-pointer op codes, same values as src/traceback.hpp:168-170. */
+#define CHSGA_OP_DELETE 0   /* (i-1, v) -> (i, v): read char not aligned to a graph char */
+#define CHSGA_OP_MATCHSUB 1 /* (i-1, u) -> (i, v), u an in-neighbour of v             */
+#define CHSGA_OP_INSERT 2   /* (i, u)   -> (i, v): graph char skipped, same read row   */
+
+/*
+ * Graph descriptor: host pointers, all owned by the caller.
+ *
+ * Char graph: V char vertices with labels[V] and the in-CSR in_off[V+1] /
+ * in_src[E] (in-neighbours of v in ascending order, as build_in_csr produces,
+ * src/align.hpp:220-242).  String graph: n_s string vertices whose chars are
+ * [str_off[s], str_off[s+1]) (the -o label offsets) and out-CSR
+ * str_out_off[n_s+1] / str_out_dst[m_str] (the -i graph).  Contraction
+ * hierarchy (the -c file; fields of the reference's PchGraph as used at
+ * src/align.hpp:294-343): CH ids 0..n_s-1, rank[og] = CH id of string og,
+ * layers layer_off[num_layers+1] (CH-id ranges, one per (component, level)),
+ * cc_off[num_cc+1] (layer-index ranges per connected component), up-edges
+ * stored at the lower endpoint u: up_off/up_v/up_w (u -> v, v higher),
+ * down-edges stored at the lower endpoint u: dn_off/dn_v/dn_w (v -> u, v
+ * higher).  up_mid / dn_mid (CH id of the contracted middle vertex of a
+ * shortcut, -1 for an original edge) may be NULL; they are used only to
+ * expand shortcuts in traceback.
+ */
+typedef struct chsga_graph_desc {
+  int64_t V;
+  const char* labels;
+  const uint64_t* in_off;
+  const uint32_t* in_src;
+  int64_t E;
+
+  int64_t n_s;
+  const uint64_t* str_off;
+  const uint64_t* str_out_off;
+  const uint32_t* str_out_dst;
+  int64_t m_str;
+
+  int64_t num_layers;
+  const uint64_t* layer_off;
+  int64_t num_cc;
+  const uint64_t* cc_off;
+  const uint32_t* rank;
+  const uint64_t* up_off;
+  const uint32_t* up_v;
+  const int32_t* up_w;
+  int64_t m_up;
+  const uint64_t* dn_off;
+  const uint32_t* dn_v;
+  const int32_t* dn_w;
+  int64_t m_dn;
+  const int32_t* up_mid;
+  const int32_t* dn_mid;
+
+  int32_t alphabet;
+  int32_t match_score, subst_score, del_score, ins_score;
+} chsga_graph_desc;
+
+typedef struct chsga_ctx chsga_ctx;
+
+/* Timing and traffic figures of the last call (all optional to read). */
+typedef struct chsga_stats {
+  double wall_ms;          /* host wall clock of the call                  */
+  double row_kernel_ms;    /* sum of row-kernel durations (profile only)    */
+  int64_t row_kernel_launches;
+  double sweep_ms;         /* sum of CH sweep + post-pass durations (profile)*/
+  int64_t cells;           /* sum over reads of len * V                     */
+  int64_t kernel_launches; /* all kernel launches of the call               */
+  int64_t rows;            /* DP rows computed (forward + recompute)        */
+  int64_t walk_fallbacks;  /* traceback rows that needed the staged walk    */
+  int64_t peak_device_bytes;
+} chsga_stats;
+
+/* Context options, chsga_ctx_set_option. */
+#define CHSGA_OPT_PROFILE 1      /* 1: time row kernels with HIP events      */
+#define CHSGA_OPT_USE_GRAPHS 2   /* 1: replay captured hipGraphs per rows     */
+#define CHSGA_OPT_TOP_LEVEL 3    /* first CH level handled by the one-group top kernel (-1 auto) */
+#define CHSGA_OPT_TILE_CHARS 4   /* target chars per row-kernel tile (0 auto) */
+
+const char* chsga_last_error(void);
+int chsga_device_count(int* count);
+
+int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out);
+void chsga_ctx_destroy(chsga_ctx* ctx);
+int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value);
+
+/* Score-only alignment of n reads, one read per launch sequence (the plain
+ * path).  scores_out[n]; per-read seconds total/sub/ins may be NULL. */
+int chsga_score_reads(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,
+                      int32_t* scores_out, double* total_sec, double* sub_sec, double* ins_sec,
+                      chsga_stats* stats);
+
+/* Last DP row of one read (V int32 values) copied to row_out; for tests. */
+int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_out);
+
+/* Batched score-only alignment: reads share slot-interleaved rows on the GPU
+ * and a device work queue refills a slot when its read ends.  order[n] (may
+ * be NULL) is the order in which reads enter the queue.  slots in {16,32,64}. */
+typedef struct chsga_batch_opts {
+  int32_t slots;
+  int32_t use_graphs;
+  const int64_t* order;
+} chsga_batch_opts;
+int chsga_score_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,
+                      const chsga_batch_opts* opts, int32_t* scores_out, chsga_stats* stats);
+
+/* Asynchronous form of chsga_score_batch: the work runs on a host worker
+ * thread; chsga_wait joins it.  The arrays must stay valid until chsga_wait. */
+int chsga_submit_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,
+                       const chsga_batch_opts* opts, int32_t* scores_out);
+int chsga_wait(chsga_ctx* ctx, chsga_stats* stats);
+
+/* One step of an alignment path: the reference's AlignOp
+ * (src/traceback.hpp:185-189). */
+typedef struct chsga_align_op {
+  int64_t row;
+  uint32_t vertex;
+  uint8_t op;
+} chsga_align_op;
+
+typedef struct chsga_path {
+  int64_t n_ops;
+  chsga_align_op* ops;
+  int32_t score;
+  uint32_t end_vertex;
+} chsga_path;
+
+/* Checkpointed traceback of one read: rows 0, k, 2k, ... are kept in HBM,
+ * segments are recomputed on the GPU and walked by score equality with the
+ * reference's predecessor priority (src/traceback.hpp:524-569).  k >= 1. */
+int chsga_traceback_read(chsga_ctx* ctx, const char* read, int64_t len, int64_t k, chsga_path** out,
+                         chsga_stats* stats);
+void chsga_path_free(chsga_path* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

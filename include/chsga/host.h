@@ -1,0 +1,101 @@
+/*
+ * chsga/host.h -- host-side C ABI: input formats, the seeded synthetic
+ * graph/read generator and the contraction-hierarchy builder.
+ *
+ * Counterparts in the reference:
+ *   chsga_host_load        <- the drivers' input stage: read_graph (-I, -i),
+ *                             read_contracted_graph (-c), the labels reader
+ *                             (src/ConSGA.cpp:159-170) and the label-offsets
+ *                             reader (src/ConSGA.cpp:175-185)
+ *   chsga_host_build_ch    <- build_pch (src/CHoverrides/build_pch.cpp:94-130,
+ *                             build_pch.hpp:912-1304), vertex-weighted
+ *   chsga_host_save        <- csr_to_adj (src/csr_to_adj.cpp:46-114) and
+ *                             write_pbbs_format (build_pch.cpp:126)
+ *   chsga_host_load_reads  <- load_reads (src/load_reads.hpp:7-40)
+ *   chsga_host_generate,
+ *   chsga_host_gen_reads   <- no counterpart (the reference ships no data)
+ */
+#ifndef CHSGA_HOST_H
+#define CHSGA_HOST_H
+
+#include <stdint.h>
+
+#include "chsga/abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chsga_dataset chsga_dataset;
+typedef struct chsga_readset chsga_readset;
+
+/* Seeded chain-of-bubbles DAG: uniform ACGT backbone, SNP bubbles, indel
+ * bubbles (50/50 deletion allele or insertion allele, each with a skip edge)
+ * and structural bubbles, at geometric gaps. */
+typedef struct chsga_graph_params {
+  uint64_t seed;
+  int64_t backbone_bp;
+  double snp_rate;   /* events per bp */
+  double indel_rate; /* events per bp */
+  int32_t indel_min, indel_max;
+  double sv_rate; /* events per bp */
+  int32_t sv_min, sv_max;
+} chsga_graph_params;
+
+typedef struct chsga_dataset_info {
+  int64_t V, E, n_s, m_str, m_up, m_dn, num_layers, num_cc, max_level, max_str_len;
+} chsga_dataset_info;
+
+const char* chsga_host_last_error(void);
+
+int chsga_host_generate(const chsga_graph_params* p, chsga_dataset** out);
+/* Builds (or rebuilds) the contraction hierarchy of the string graph with
+ * vertex weights len(s)-1 (shortcut weight w1 + w2 + len(mid) - 1). */
+int chsga_host_build_ch(chsga_dataset* ds, uint64_t seed);
+/* Drop-in loader.  ch_path may be NULL (then the CH is built). */
+int chsga_host_load(const char* char_adj, const char* str_adj, const char* ch_path, const char* labels,
+                    const char* offsets, chsga_dataset** out);
+/* Writes <prefix>.char.adj, .str.adj, .ch, .labels, .label_offsets. */
+int chsga_host_save(const chsga_dataset* ds, const char* prefix);
+int chsga_host_save_ch(const chsga_dataset* ds, const char* path);
+int chsga_host_describe(const chsga_dataset* ds, int32_t alphabet, chsga_graph_desc* out);
+int chsga_host_info(const chsga_dataset* ds, chsga_dataset_info* out);
+void chsga_host_free(chsga_dataset* ds);
+
+/* Read generation: windows of random source-to-sink paths (uniform branch
+ * choice) with i.i.d. substitutions, insertions, deletions and N characters.
+ * len_mode 0: fixed len_a; 1: log-normal, median len_a, sigma len_b, clipped
+ * to [len_min, len_max]; 2: mixture: with probability mix_frac a uniform
+ * length in [len_min, len_max] with error rates scaled by err_scale_a,
+ * otherwise uniform in [len2_min, len2_max] scaled by err_scale_b. */
+typedef struct chsga_read_params {
+  uint64_t seed;
+  int64_t n_reads;
+  int32_t len_mode;
+  double len_a, len_b;
+  int64_t len_min, len_max;
+  double sub_rate, ins_rate, del_rate, n_rate;
+  double mix_frac;
+  int64_t len2_min, len2_max;
+  double err_scale_a, err_scale_b;
+} chsga_read_params;
+
+int chsga_host_gen_reads(const chsga_dataset* ds, const chsga_read_params* p, chsga_readset** out);
+/* FASTA/FASTQ, optionally gzip-compressed, or one sequence per line.
+ * Sequences are upper-cased.  cap > 0 stops after cap reads (the reference
+ * stops after 5, src/load_reads.hpp:5). */
+int chsga_host_load_reads(const char* path, int64_t cap, chsga_readset** out);
+int chsga_host_save_reads(const chsga_readset* rs, const char* path);
+int64_t chsga_reads_count(const chsga_readset* rs);
+const char* chsga_reads_data(const chsga_readset* rs);
+const int64_t* chsga_reads_offsets(const chsga_readset* rs);
+void chsga_reads_free(chsga_readset* rs);
+
+/* Direct array access for tests and tools (pointers into the dataset). */
+const char* chsga_ds_labels(const chsga_dataset* ds);
+const uint64_t* chsga_ds_str_off(const chsga_dataset* ds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
