@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""GCUPS benchmark of the ConSGA graph-DP hot path on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic reads of the
+c3 workload (20 Mbp seeded chain-of-bubbles DAG, ~22M char vertices; reads
+log-normal median 3 kbp clipped to 1-30 kbp, 2% error, 1% N, mismatchN).
+  value      GCUPS of the batched path (slot-interleaved rows, device work
+             queue, reads packed in order of the learned cost model), whole
+             job, inputs resident in HBM;
+  plain      GCUPS of the plain path (one read per launch sequence) on the
+             same graph;
+  roofline   the row kernel (k_row_tile): algorithmic bytes per launch over
+             its mean duration from HIP events on the engine's stream;
+  cpu_baseline  the oracle (C++/OpenMP restatement of ConSGA) on a bounded
+             sample of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "GCUPS (read chars x graph chars / s) on 1 GPU: batched+learned vs plain, exact"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def row_kernel_bytes(info: dict) -> int:
+    """Algorithmic bytes of one k_row_tile launch (one DP row, one read):
+    read H 4V + write D 4V + labels V + in_off 4(V+1) + in_src 4E +
+    first-char bit mask V/8 + F and Lst writes 8 n_s."""
+    V, E, n_s = info["V"], info["E"], info["n_s"]
+    return 4 * V + 4 * V + V + 4 * (V + 1) + 4 * E + V // 8 + 8 * n_s
+
+
+def row_bytes_survey(info: dict) -> int:
+    """Whole-row algorithmic bytes: 4V + 4V + V + 4(V+1) + 4E + 12 m_CH + 8 n_s."""
+    V, E, n_s = info["V"], info["E"], info["n_s"]
+    return 4 * V + 4 * V + V + 4 * (V + 1) + 4 * E + 12 * (info["m_up"] + info["m_dn"]) + 8 * n_s
+
+
+def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
+    from oracle import oracle as O
+    threads = O.threads(0)
+    d = ds.desc(wl.alphabet)
+    V = d.V
+    # bounded sample: a prefix of read 0, sized from a short probe
+    probe_rows = 16
+    r0 = reads[0]
+    t = time.perf_counter()
+    O.consga_scores(d, type(reads).from_list([r0[:probe_rows]]))
+    per_row = (time.perf_counter() - t) / probe_rows
+    rows = int(max(32, min(len(r0), budget_s / max(per_row, 1e-6))))
+    sample = type(reads).from_list([r0[:rows]])
+    t = time.perf_counter()
+    cpu_score = O.consga_scores(d, sample)
+    dt = time.perf_counter() - t
+    gpu_score = gpu_engine.align_combo(sample)
+    return {
+        "value": round(rows * V / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+        "sample": f"read 0 of the c3 read set truncated to its first {rows} chars ({rows} DP rows x {V} chars), "
+                  f"{dt:.1f} s on {threads} OpenMP threads",
+        "score_cpu": int(cpu_score[0]), "score_gpu": int(gpu_score[0]), "exact": bool(cpu_score[0] == gpu_score[0]),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--batch-reads", type=int, default=256, help="reads per batched step")
+    ap.add_argument("--plain-reads", type=int, default=4, help="reads per plain step")
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="only run a torch copy of N MiB (PMC scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the path shards by reads and no multi-GPU figure is taken "
+              "(use --gpus 1)", file=sys.stderr)
+        return 2
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} MiB x5")
+        return 0
+
+    from chsga_amd import WORKLOADS, Engine
+    from chsga_amd import native as N
+    from chsga_amd.scheduler import plan_order
+
+    wl = WORKLOADS[args.workload]
+    t = time.perf_counter()
+    ds = wl.dataset()
+    info = ds.info()
+    log(f"[bench] {wl.name} graph: V={info['V']} E={info['E']} n_s={info['n_s']} levels={info['num_layers']} "
+        f"({time.perf_counter() - t:.1f}s)")
+    n_needed = max(args.batch_reads, args.plain_reads) * (args.steps + args.warmup)
+    reads = wl.readset(ds, n_reads=n_needed)
+    eng = Engine(ds, wl.alphabet)
+    eng.set_option(N.OPT_PROFILE, 0)
+    V = info["V"]
+    result = {}
+
+    def step_reads(k: int, per: int):
+        return reads.subset(range(k * per, (k + 1) * per))
+
+    # ---- plain path: one read per launch sequence -------------------------
+    plain_gcups = None
+    if args.mode in ("both", "plain"):
+        for w in range(args.warmup):
+            eng.align_combo(step_reads(w, args.plain_reads))
+        cells = 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            rs = step_reads(args.warmup + k, args.plain_reads)
+            eng.align_combo(rs)
+            cells += int(rs.lengths().sum()) * V
+        dt = time.perf_counter() - t0
+        plain_gcups = cells / dt / 1e9
+        result["plain"] = {"gcups": round(plain_gcups, 3), "ms_per_step": round(dt * 1e3 / args.steps, 2),
+                           "reads_per_step": args.plain_reads}
+        log(f"[bench] plain: {plain_gcups:.2f} GCUPS ({dt:.2f}s, {args.steps} steps)")
+
+    # ---- batched path -------------------------------------------------------
+    batch_gcups = None
+    if args.mode in ("both", "batch"):
+        for w in range(args.warmup):
+            rs = step_reads(w, args.batch_reads)
+            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds))
+        cells = 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            rs = step_reads(args.warmup + k, args.batch_reads)
+            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds))
+            cells += int(rs.lengths().sum()) * V
+        dt = time.perf_counter() - t0
+        batch_gcups = cells / dt / 1e9
+        result["batch"] = {"gcups": round(batch_gcups, 3), "ms_per_step": round(dt * 1e3 / args.steps, 2),
+                           "reads_per_step": args.batch_reads, "slots": args.slots}
+        log(f"[bench] batched: {batch_gcups:.2f} GCUPS ({dt:.2f}s)")
+
+    # ---- roofline of the row kernel (profiled, separate short run) -------
+    eng.set_option(N.OPT_PROFILE, 1)
+    prof = reads.subset([0])
+    eng.align_combo(prof)
+    st = eng.last_stats
+    eng.set_option(N.OPT_PROFILE, 0)
+    avg_ms = st["row_kernel_ms"] / max(1, st["row_kernel_launches"])
+    kb = row_kernel_bytes(info)
+    achieved = kb / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_row_tile", "kernel_avg_us": round(avg_ms * 1e3, 2),
+                "bytes_per_launch": kb, "launches_timed": st["row_kernel_launches"],
+                "sweep_avg_us_per_row": round(st["sweep_ms"] * 1e3 / max(1, st["row_kernel_launches"]), 2)}
+    log(f"[bench] row kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ds, wl, reads, args.cpu_budget, eng)
+        log(f"[bench] cpu baseline: {cpu['value']} GCUPS on {cpu['cores']} threads, exact={cpu['exact']}")
+
+    value = batch_gcups if batch_gcups is not None else plain_gcups
+    ms = result.get("batch", result.get("plain"))["ms_per_step"]
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded graph and reads)",
+        "config": {"workload": f"{wl.name}: 20 Mbp seeded DAG (SNP 1/25, indels 1-50 at 1/200), V={V} chars, "
+                               f"n_s={info['n_s']}; reads log-normal median 3 kbp (1-30 kbp), 2% error, 1% N, "
+                               f"mismatchN, score-only",
+                   "batch_reads": args.batch_reads, "plain_reads": args.plain_reads, "slots": args.slots},
+        "plain_gcups": None if plain_gcups is None else round(plain_gcups, 3),
+        "batched_gcups": None if batch_gcups is None else round(batch_gcups, 3),
+        "speedup_batched_vs_plain": (round(batch_gcups / plain_gcups, 2) if batch_gcups and plain_gcups else None),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

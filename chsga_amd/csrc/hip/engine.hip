@@ -189,6 +189,9 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   }
   c->j_top = jt;
 
+  // every check above is host-only; the device is touched from here on
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   hipStream_t s = c->stream;
   std::vector<uint8_t> labels(g->labels, g->labels + V);
   upload(c->d_labels, labels, s);
@@ -348,8 +351,6 @@ int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out) {
     c = new chsga_ctx();
     try {
       c->device = device;
-      hip_check(hipSetDevice(device), "hipSetDevice");
-      hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
       build_context(c, g);
     } catch (...) {
       delete c;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -32,6 +33,8 @@ struct DevMem {
 
 }  // namespace chsga_engine
 
+struct chsga_batch_state;
+
 struct chsga_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -42,6 +45,7 @@ struct chsga_ctx {
   int32_t J = 0, j_top = 0;
   int64_t top_level_opt = -1;
   int64_t tile_chars = 2048;
+  int64_t batch_tile_chars = 512;
   std::vector<uint32_t> h_lvl_off, h_up_off, h_dn_off;
   // host copies for traceback
   std::string h_labels;
@@ -62,6 +66,8 @@ struct chsga_ctx {
   std::thread worker;
   int worker_status = 0;
   chsga_stats worker_stats{};
+  std::string worker_error;
+  std::shared_ptr<chsga_batch_state> batch;
 
   ~chsga_ctx();
   void launch_init(int32_t* H, uint8_t c0);
@@ -69,4 +75,6 @@ struct chsga_ctx {
   void launch_insertion(int32_t* D);
   int forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows);
   void flush_profile();
+  void launch_tick(int32_t* H, int32_t* D, const void* reads_dev, const int64_t* rd_off_dev,
+                   const int32_t* order_dev, int32_t n_reads, int32_t* scores_dev, int32_t active_slots);
 };

@@ -1,0 +1,138 @@
+"""GPU parity: the HIP engine against the oracle, bit-exact (int32 scores and rows).
+
+Covers both alphabets, the plain path (one read per launch sequence) and the
+batched path (slot-interleaved rows + device work queue), uneven read lengths,
+one-char reads, N characters, multi-component graphs, long string vertices,
+the committed golden fixtures, and at full c3 size the size-independent
+properties (plain == batched, 0 <= score <= |r|, error-free reads score 0).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from chsga_amd import ALPHABET_MATCH_N, ALPHABET_MISMATCH_N, Dataset, Engine, ReadSet, WORKLOADS, small_workload
+from chsga_amd.scheduler import plan_order
+from oracle import oracle as O
+from tests import refmodels as R
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("alphabet", [ALPHABET_MISMATCH_N, ALPHABET_MATCH_N])
+def test_last_rows_match_oracle(alphabet):
+    ds, rs = small_workload(seed=11, n_reads=6, length=90, n_rate=0.05)
+    d = ds.desc(alphabet)
+    eng = Engine(ds, alphabet)
+    for i in range(len(rs)):
+        _say("last row", alphabet, i, len(rs[i]))
+        assert np.array_equal(eng.last_row(rs[i]), O.consga_last_row(d, rs[i]))
+    eng.close()
+
+
+@pytest.mark.parametrize("alphabet", [ALPHABET_MISMATCH_N, ALPHABET_MATCH_N])
+def test_scores_plain_and_batched_match_oracle(alphabet):
+    ds, rs = small_workload(seed=12, n_reads=90, length=140, n_rate=0.02)
+    # uneven lengths including one-char reads
+    reads = [rs[i][: 1 + (i * 37) % 140] for i in range(len(rs))] + [b"N" * 9, b"A", b"Z" * 3]
+    rs = ReadSet.from_list(reads)
+    ref = O.consga_scores(ds.desc(alphabet), rs)
+    eng = Engine(ds, alphabet)
+    _say("plain", alphabet)
+    assert np.array_equal(eng.align_combo(rs), ref)
+    _say("batched fifo")
+    assert np.array_equal(eng.score_batch(rs), ref)
+    _say("batched planned + graphs")
+    assert np.array_equal(eng.score_batch(rs, order=plan_order(rs, ds), use_graphs=True), ref)
+    _say("batched 16 slots")
+    assert np.array_equal(eng.score_batch(rs, slots=16), ref)
+    eng.close()
+
+
+def test_batch_calls_back_to_back_and_async():
+    ds, rs = small_workload(seed=13, n_reads=70, length=60)
+    ref = O.consga_scores(ds.desc(), rs)
+    eng = Engine(ds)
+    a = eng.score_batch(rs)
+    b = eng.score_batch(rs.subset(range(10)))
+    assert np.array_equal(a, ref) and np.array_equal(b, ref[:10])
+    eng.close()
+
+
+def test_multi_component_and_long_strings(tmp_path):
+    rng = np.random.default_rng(4)
+    strings, edges = [], []
+    for comp in range(3):  # three components, one with a 3000-char string vertex
+        base = len(strings)
+        s, e = R.random_string_dag(rng, 25, 6, p_edge=0.08)
+        if comp == 1:
+            s[5] = R.rbytes(rng, 3000)
+        strings += s
+        edges += [(a + base, b + base) for a, b in e]
+    p = R.write_dataset(tmp_path, strings, edges, "mc")
+    ds = Dataset.load(p["char.adj"], p["str.adj"], None, p["labels"], p["label_offsets"])
+    assert ds.info()["num_cc"] == 3
+    reads = ReadSet.from_list([R.rbytes(rng, int(rng.integers(1, 80))) for _ in range(40)] +
+                              [strings[25 + 5][100:400]])
+    ref = O.consga_scores(ds.desc(), reads)
+    eng = Engine(ds)
+    assert np.array_equal(eng.align_combo(reads), ref)
+    assert np.array_equal(eng.score_batch(reads), ref)
+    for i in (0, len(reads) - 1):
+        assert np.array_equal(eng.last_row(reads[i]), O.consga_last_row(ds.desc(), reads[i]))
+    eng.close()
+
+
+def test_golden_fixtures():
+    files = sorted(GOLDEN.glob("*.json"))
+    assert files, "golden fixtures missing"
+    for f in files:
+        g = json.loads(f.read_text())
+        _say("golden", f.name)
+        gp = dict(g["graph"])
+        for k in ("indel_len", "sv_len"):
+            if k in gp:
+                gp[k] = tuple(gp[k])
+        ds = Dataset.generate(**gp)
+        rs = ReadSet.from_list([r.encode() for r in g["reads"]])
+        eng = Engine(ds, g["alphabet"])
+        assert list(eng.align_combo(rs)) == g["scores"]
+        assert list(eng.score_batch(rs)) == g["scores"]
+        row = eng.last_row(rs[0])
+        assert int(row.sum()) == g["row0_sum"] and int(row.argmin()) == g["row0_argmin"]
+        eng.close()
+
+
+def test_c1_all_reads_match_oracle(c1):
+    w, ds, rs = c1
+    ref = O.consga_scores(ds.desc(w.alphabet), rs)
+    eng = Engine(ds, w.alphabet)
+    assert np.array_equal(eng.align_combo(rs), ref)
+    assert np.array_equal(eng.score_batch(rs, order=plan_order(rs, ds)), ref)
+    eng.close()
+
+
+def test_c3_full_size_properties():
+    """Full-size graph: scores agree between the two GPU paths, lie in [0, |r|],
+    and error-free reads spelled along graph paths score 0."""
+    w = WORKLOADS["c3"]
+    ds = w.dataset()
+    rs = w.readset(ds, n_reads=3)
+    short = ReadSet.from_list([rs[i][:400] for i in range(3)])
+    clean = ReadSet.generate(ds, seed=99, n_reads=2, length=300)
+    both = ReadSet.from_list([short[i] for i in range(3)] + [clean[0], clean[1]])
+    eng = Engine(ds, w.alphabet)
+    _say("c3 plain")
+    p = eng.align_combo(both)
+    _say("c3 batched")
+    b = eng.score_batch(both)
+    assert np.array_equal(p, b)
+    assert (p >= 0).all() and (p <= both.lengths()).all()
+    assert p[3] == 0 and p[4] == 0
+    eng.close()

@@ -1,0 +1,203 @@
+"""Pins of the CPU oracle (no GPU): closed forms, brute force, invariants.
+
+The reference holds no tests, fixtures or golden vectors for this path, so the
+oracle is pinned by what the mathematics fixes:
+  * sigma against its truth table (src/ParSGAoverrides/csr_align.hpp:41-50);
+  * O1 (exact closure) against exhaustive path enumeration on tiny DAGs and
+    against the textbook Levenshtein form on chains;
+  * O2 (ConSGA) equals O1 whenever every string vertex has one char, where the
+    CH sweep is an exact PHAST closure; O2 >= O1 always (every O2 value is
+    the cost of some path); O2's row 0 has no insertion closure;
+  * score invariants: 0 <= score <= |r|, a read spelled along a graph path
+    scores 0, an all-N read under mismatchN on an N-free graph scores |r|,
+    a one-char read scores min sigma.
+"""
+import numpy as np
+import pytest
+
+from chsga_amd import ALPHABET_MATCH_N, ALPHABET_MISMATCH_N, Dataset, ReadSet
+from oracle import oracle as O
+from tests import refmodels as R
+
+BASES = b"ACGTN"
+
+
+def test_sigma_truth_table():
+    for a in BASES:
+        for b in BASES:
+            ab, bb = bytes([a]), bytes([b])
+            exp5 = 0 if a == b else 1
+            exp4 = 0 if (a == ord("N") or b == ord("N") or a == b) else 1
+            assert O.sigma(ALPHABET_MISMATCH_N, ab, bb) == exp5
+            assert O.sigma(ALPHABET_MATCH_N, ab, bb) == exp4
+    # N vs N matches under both alphabets
+    assert O.sigma(ALPHABET_MISMATCH_N, b"N", b"N") == 0 and O.sigma(ALPHABET_MATCH_N, b"N", b"N") == 0
+    # labels are not upper-cased: 'a' != 'A'
+    assert O.sigma(ALPHABET_MISMATCH_N, b"a", b"A") == 1
+
+
+def _load(tmp_path, strings, edges, name="g"):
+    p = R.write_dataset(tmp_path, strings, edges, name)
+    return Dataset.load(p["char.adj"], p["str.adj"], None, p["labels"], p["label_offsets"])
+
+
+def test_refmodels_agree_on_tiny_dags():
+    """The DP over alignment states equals exhaustive path enumeration."""
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        strings, edges = R.random_string_dag(rng, int(rng.integers(1, 5)), 3)
+        labels, off, ins = R.expand(strings, edges)
+        if len(labels) > 10:
+            continue
+        read = R.rbytes(rng, int(rng.integers(1, 5)))
+        assert R.best_path_score(len(labels), labels, ins, read) == R.enumerate_paths_score(len(labels), labels, ins, read)
+
+
+def test_exact_oracle_matches_brute_force(tmp_path):
+    rng = np.random.default_rng(5)
+    checked = 0
+    for t in range(60):
+        strings, edges = R.random_string_dag(rng, int(rng.integers(1, 7)), 3)
+        labels, off, ins = R.expand(strings, edges)
+        ds = _load(tmp_path, strings, edges, f"g{t}")
+        reads = [R.rbytes(rng, int(rng.integers(1, 7))) for _ in range(3)]
+        rs = ReadSet.from_list(reads)
+        ex = O.exact_scores(ds.desc(), rs)
+        for r, e in zip(reads, ex):
+            exp = R.enumerate_paths_score(len(labels), labels, ins, r) if len(labels) <= 12 and len(r) <= 6 \
+                else R.best_path_score(len(labels), labels, ins, r)
+            assert e == exp, (strings, edges, r)
+            checked += 1
+    assert checked >= 100
+
+
+def test_chain_matches_textbook_levenshtein(tmp_path):
+    """One chain: O1 and O2 equal sigma(G[a], r0) + Levenshtein(r[1:], G[a+1..b]) minimised
+    over windows; with one-char strings the chain exercises the CH sweep too."""
+    rng = np.random.default_rng(3)
+    for t in range(12):
+        n = int(rng.integers(4, 14))
+        g = R.rbytes(rng, n)
+        for split in ("one", "chars"):
+            strings = [g] if split == "one" else [g[i:i + 1] for i in range(n)]
+            edges = [] if split == "one" else [(i, i + 1) for i in range(n - 1)]
+            ds = _load(tmp_path, strings, edges, f"c{t}{split}")
+            reads = [R.rbytes(rng, int(rng.integers(1, 9))) for _ in range(4)]
+            reads.append(g[2:n - 1])
+            rs = ReadSet.from_list(reads)
+            exp = [R.chain_score(g, r) for r in reads]
+            assert list(O.exact_scores(ds.desc(), rs)) == exp
+            assert list(O.consga_scores(ds.desc(), rs)) == exp
+
+
+def test_consga_equals_exact_on_single_char_strings(tmp_path):
+    rng = np.random.default_rng(9)
+    for t in range(25):
+        strings, edges = R.random_string_dag(rng, int(rng.integers(2, 40)), 1, p_edge=0.15)
+        ds = _load(tmp_path, strings, edges, f"s{t}")
+        rs = ReadSet.from_list([R.rbytes(rng, int(rng.integers(1, 20))) for _ in range(5)])
+        assert (O.consga_scores(ds.desc(), rs) == O.exact_scores(ds.desc(), rs)).all()
+        d = ds.desc()
+        for i in range(len(rs)):
+            assert (O.consga_last_row(d, rs[i]) == O.exact_last_row(d, rs[i])).all()
+
+
+def test_consga_never_below_exact(tmp_path):
+    """Every ConSGA value is the cost of some path, so it bounds the exact optimum
+    from above; disagreements (the stale last() of multi-char vertices) are counted."""
+    rng = np.random.default_rng(21)
+    disagree = 0
+    for t in range(25):
+        strings, edges = R.random_string_dag(rng, int(rng.integers(2, 30)), 4, p_edge=0.1)
+        ds = _load(tmp_path, strings, edges, f"m{t}")
+        d = ds.desc()
+        for _ in range(3):
+            r = R.rbytes(rng, int(rng.integers(1, 25)))
+            a, b = O.consga_last_row(d, r), O.exact_last_row(d, r)
+            assert (a >= b).all()
+            disagree += int((a != b).any())
+    assert disagree < 75
+
+
+def test_row0_has_no_insertion_closure(tmp_path):
+    """Row 0 is sigma only (src/align.hpp:205-210, 421).  Under unit costs a
+    closure could never lower row 0, so the check uses ins = 0, where it would."""
+    from chsga_amd import native as N
+    ds = _load(tmp_path, [b"A", b"C", b"G"], [(0, 1), (1, 2)], "r0")
+    d = N.GraphDesc.from_buffer_copy(ds.desc())
+    d.ins_score = 0
+    assert list(O.consga_last_row(d, b"A")) == [0, 1, 1]
+    # on row 1 the closure does apply: 'AC' ends at C with 0, then G gets 0 + ins = 0
+    assert list(O.consga_last_row(d, b"AC")) == [1, 0, 0]
+
+
+def test_invariants_on_generated_graph(small):
+    ds, rs = small
+    d = ds.desc()
+    sc = O.consga_scores(d, rs)
+    lens = rs.lengths()
+    assert (sc >= 0).all() and (sc <= lens).all()
+    # a read spelled along a path of the graph scores 0
+    clean = ReadSet.generate(ds, seed=77, n_reads=4, length=60)  # error-free windows of graph paths
+    assert list(O.consga_scores(d, clean)) == [0, 0, 0, 0]
+    # all-N read, mismatchN, N-free graph: every char mismatches -> |r| (deletions cost the same)
+    nread = ReadSet.from_list([b"N" * 17])
+    assert list(O.consga_scores(d, nread)) == [17]
+    # matchN: N matches anything -> 0
+    assert list(O.consga_scores(ds.desc(ALPHABET_MATCH_N), nread)) == [0]
+    # one-char reads: min over sigma
+    one = ReadSet.from_list([b"A", b"Z"])
+    assert list(O.consga_scores(d, one)) == [0, 1]
+
+
+def test_traceback_oracle_paths_are_valid(small):
+    ds, rs = small
+    d = ds.desc()
+    scores = O.consga_scores(d, rs)
+    for i in range(len(rs)):
+        sc, ops = O.traceback(d, rs[i])
+        assert sc == scores[i]
+        assert ops is not None
+        rc, bad = O.validate_path(d, rs[i], ops, sc)
+        assert rc == 0, (i, rc, bad)
+
+
+def test_validator_rejects_planted_mistakes(small):
+    """A plausible traceback bug must fail the validator: a wrong op, a wrong
+    vertex, a dropped step, an off-by-one score."""
+    ds, rs = small
+    d = ds.desc()
+    sc, ops = O.traceback(d, rs[0])
+    assert O.validate_path(d, rs[0], ops, sc + 1)[0] == 6
+    bad = ops.copy()
+    k = len(bad) // 2
+    bad[k]["vertex"] = (int(bad[k]["vertex"]) + 7) % d.V
+    assert O.validate_path(d, rs[0], bad, sc)[0] != 0
+    assert O.validate_path(d, rs[0], np.delete(ops, k), sc)[0] != 0
+    bad = ops.copy()
+    bad[0]["op"] = 2
+    assert O.validate_path(d, rs[0], bad, sc)[0] == 2
+    assert O.validate_path(d, rs[0], ops[:-1], sc)[0] != 0
+
+
+def test_golden_fixtures_reproduce():
+    """The committed fixtures: the seeded generator rebuilds the same graph and
+    the oracle the same scores (O1 <= O2 on every read)."""
+    import json
+    from pathlib import Path
+    files = sorted((Path(__file__).resolve().parent / "golden").glob("*.json"))
+    assert len(files) >= 3
+    for f in files:
+        g = json.loads(f.read_text())
+        gp = dict(g["graph"])
+        for k in ("indel_len", "sv_len"):
+            if k in gp:
+                gp[k] = tuple(gp[k])
+        ds = Dataset.generate(**gp)
+        assert ds.info() == g["info"]
+        rs = ReadSet.from_list([r.encode() for r in g["reads"]])
+        d = ds.desc(g["alphabet"])
+        assert list(O.consga_scores(d, rs)) == g["scores"]
+        assert all(e <= s for e, s in zip(g["exact_scores"], g["scores"]))
+        row = O.consga_last_row(d, rs[0])
+        assert int(row.sum()) == g["row0_sum"] and int(row.argmin()) == g["row0_argmin"]
