@@ -77,11 +77,10 @@ def build(verbose: bool = False) -> None:
         ("ConSGA_traceback_mismatchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=5"]),
         ("ConSGA_traceback_matchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=4"]),
         ("build_pch", "build_pch.cpp", []),
-        ("chsga_gen", "chsga_gen.cpp", []),
     ]:
         if (drv / src).exists():
-            stage2.append([CXX, *CXX_FLAGS, *defs, "-I", INC, drv / src, "-L", LIB, "-lchsga", "-lchsga_host",
-                           rpath, "-o", BIN / name])
+            libs = ["-lchsga_host"] if name == "build_pch" else ["-lchsga", "-lchsga_host"]
+            stage2.append([CXX, *CXX_FLAGS, *defs, "-I", INC, drv / src, "-L", LIB, *libs, rpath, "-o", BIN / name])
     if stage2:
         _run(stage2, verbose)
 

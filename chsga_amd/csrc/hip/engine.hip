@@ -207,6 +207,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_up_in, up_in, s);
   upload(c->d_dn_off, dn_off, s);
   upload(c->d_dn_in, dn_in, s);
+  c->ensure_staged_tables(g);
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);
   c->d_F.alloc(sizeof(int32_t) * n_s);
@@ -229,34 +230,40 @@ void chsga_ctx::launch_row(const int32_t* H, int32_t* D, uint8_t rc) {
   if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
 }
 
-void chsga_ctx::launch_insertion(int32_t* D) {
+void chsga_ctx::launch_sweeps(int mode) {
   const hipStream_t s = stream;
   int32_t* F = (int32_t*)d_F.p;
   const int32_t* L = (const int32_t*)d_Lst.p;
-  for (int32_t j = 1; j < j_top; ++j) {
-    const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
-    if (h_up_off[k1] == h_up_off[k0]) continue;
-    hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F, L,
-                       (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p, k0, k1,
-                       isc);
-    launches += 1;
-  }
+  if (mode & 1)
+    for (int32_t j = 1; j < j_top; ++j) {
+      const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
+      if (h_up_off[k1] == h_up_off[k0]) continue;
+      hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F,
+                         L, (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p, k0, k1,
+                         isc);
+      launches += 1;
+    }
   if (j_top < J) {
     hipLaunchKernelGGL(k_sweep_top, dim3(1), dim3(1024), 0, s, F, L, (const uint32_t*)d_sw_vert.p,
                        (const uint32_t*)d_lvl_off.p, j_top, J, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p,
-                       (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, isc);
+                       (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, isc, int32_t(mode));
     launches += 1;
   }
-  for (int32_t j = std::min(j_top, J) - 1; j >= 0; --j) {
-    const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
-    if (h_dn_off[k1] == h_dn_off[k0]) continue;
-    hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F, L,
-                       (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, k0, k1,
-                       isc);
-    launches += 1;
-  }
-  hipLaunchKernelGGL(k_post, dim3(unsigned((n_s + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0, s, D,
-                     (const int32_t*)F, (const uint32_t*)d_str_first.p, n_s, isc);
+  if (mode & 2)
+    for (int32_t j = std::min(j_top, J) - 1; j >= 0; --j) {
+      const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
+      if (h_dn_off[k1] == h_dn_off[k0]) continue;
+      hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F,
+                         L, (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, k0, k1,
+                         isc);
+      launches += 1;
+    }
+}
+
+void chsga_ctx::launch_insertion(int32_t* D) {
+  launch_sweeps(3);
+  hipLaunchKernelGGL(k_post, dim3(unsigned((n_s + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0, stream, D,
+                     (const int32_t*)d_F.p, (const uint32_t*)d_str_first.p, n_s, isc);
   launches += 1;
 }
 
@@ -388,6 +395,9 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         ctx->top_level_opt = value;
         if (value >= 0) ctx->j_top = int32_t(std::min<int64_t>(value, ctx->J));
         break;
+      case CHSGA_OPT_STAGED_WALK:
+        ctx->force_staged = value != 0;
+        break;
       case CHSGA_OPT_TILE_CHARS:
         throw std::invalid_argument("tile size is fixed at context creation");
       default:
@@ -475,3 +485,4 @@ int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_o
 
 #include "batch.hip.inc"
 #include "traceback.hip.inc"
+#include "mlp.hip.inc"

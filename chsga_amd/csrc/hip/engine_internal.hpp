@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,12 @@ struct DevMem {
 
 struct chsga_batch_state;
 
+struct OgEdge {
+  uint32_t v;
+  int32_t w;
+  int32_t mid;
+};
+
 struct chsga_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -50,13 +57,16 @@ struct chsga_ctx {
   // host copies for traceback
   std::string h_labels;
   std::vector<uint32_t> h_str_off, h_in_off, h_in_src, h_sin_off, h_sin_src;
+  bool has_mid = false;
+  std::vector<std::vector<OgEdge>> h_up_og, h_dn_og, h_upin_og;  // CH edges by original string id
+  chsga_engine::DevMem d_sin_off, d_sin_src;
   // device graph
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
   // options and counters
-  bool profile = false, use_graphs = false;
+  bool profile = false, use_graphs = false, force_staged = false;
   int64_t launches = 0, rows = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -73,6 +83,11 @@ struct chsga_ctx {
   void launch_init(int32_t* H, uint8_t c0);
   void launch_row(const int32_t* H, int32_t* D, uint8_t rc);
   void launch_insertion(int32_t* D);
+  void launch_sweeps(int mode);  // bit 0: up levels, bit 1: down levels
+  void ensure_traceback_tables();
+  void ensure_staged_tables(const chsga_graph_desc* g);
+  void staged_walk_row(const int32_t* prow_dev, const char* read, int64_t i, int64_t v0,
+                       std::vector<chsga_align_op>& rev, int64_t* v_out);
   int forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows);
   void flush_profile();
   void launch_tick(int32_t* H, int32_t* D, const void* reads_dev, const int64_t* rd_off_dev,

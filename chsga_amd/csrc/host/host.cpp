@@ -600,6 +600,30 @@ int chsga_host_save_ch(const chsga_dataset* ds, const char* path) {
   CHSGA_GUARD(write_ch(path, ds->ch))
 }
 
+int chsga_host_build_ch_file(const char* str_adj, const char* label_offsets, const char* out_ch, uint64_t seed) {
+  if (!str_adj || !out_ch) {
+    set_error("missing input path");
+    return CHSGA_EINVAL;
+  }
+  CHSGA_GUARD({
+    chsga_dataset ds;
+    read_adj(str_adj, ds.n_s, ds.str_out_off, ds.str_out_dst);
+    if (label_offsets) {
+      const std::string s = slurp(label_offsets);
+      Tok t(s);
+      uint64_t v;
+      while (t.next_u64(v)) ds.str_off.push_back(v);
+      if (int64_t(ds.str_off.size()) != ds.n_s + 1)
+        throw std::runtime_error("label offsets: expected n+1 values for the string graph");
+    } else {
+      ds.str_off.resize(ds.n_s + 1);
+      for (int64_t i = 0; i <= ds.n_s; ++i) ds.str_off[i] = uint64_t(i);
+    }
+    build_ch(ds, seed);
+    write_ch(out_ch, ds.ch);
+  })
+}
+
 int chsga_host_describe(const chsga_dataset* ds, int32_t alphabet, chsga_graph_desc* g) {
   if (!ds || !g) {
     set_error("null argument");

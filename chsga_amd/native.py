@@ -18,7 +18,7 @@ CHSGA_OK = 0
 ALPHABET_MATCH_N = 4
 ALPHABET_MISMATCH_N = 5
 OP_DELETE, OP_MATCHSUB, OP_INSERT = 0, 1, 2
-OPT_PROFILE, OPT_USE_GRAPHS, OPT_TOP_LEVEL, OPT_TILE_CHARS = 1, 2, 3, 4
+OPT_PROFILE, OPT_USE_GRAPHS, OPT_TOP_LEVEL, OPT_TILE_CHARS, OPT_STAGED_WALK = 1, 2, 3, 4, 5
 
 u64p = C.POINTER(C.c_uint64)
 u32p = C.POINTER(C.c_uint32)
@@ -105,6 +105,17 @@ ENGINE_SYMBOLS = {
     "chsga_traceback_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int64, C.POINTER(C.POINTER(Path_)),
                                        C.POINTER(Stats)]),
     "chsga_path_free": (None, [C.POINTER(Path_)]),
+    "chsga_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    "chsga_mlp_destroy": (None, [C.c_void_p]),
+    "chsga_mlp_num_params": (C.c_int64, [C.c_void_p]),
+    "chsga_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
+    "chsga_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
+    "chsga_mlp_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)]),
+    "chsga_mlp_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "chsga_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int32,
+                                  C.c_int32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
 }
 
 HOST_SYMBOLS = {
@@ -115,6 +126,7 @@ HOST_SYMBOLS = {
                                   C.POINTER(C.c_void_p)]),
     "chsga_host_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "chsga_host_save_ch": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "chsga_host_build_ch_file": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64]),
     "chsga_host_describe": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(GraphDesc)]),
     "chsga_host_info": (C.c_int, [C.c_void_p, C.POINTER(DatasetInfo)]),
     "chsga_host_free": (None, [C.c_void_p]),

@@ -128,6 +128,7 @@ typedef struct chsga_stats {
 #define CHSGA_OPT_USE_GRAPHS 2   /* 1: replay captured hipGraphs per rows     */
 #define CHSGA_OPT_TOP_LEVEL 3    /* first CH level handled by the one-group top kernel (-1 auto) */
 #define CHSGA_OPT_TILE_CHARS 4   /* target chars per row-kernel tile (0 auto) */
+#define CHSGA_OPT_STAGED_WALK 5  /* 1: walk every traceback row through its stages (validation) */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);
@@ -183,6 +184,27 @@ typedef struct chsga_path {
 int chsga_traceback_read(chsga_ctx* ctx, const char* read, int64_t len, int64_t k, chsga_path** out,
                          chsga_stats* stats);
 void chsga_path_free(chsga_path* p);
+
+/*
+ * Learned cost model (no counterpart in the reference, which schedules
+ * nothing): an FP32 multilayer perceptron in -> h1 -> h2 -> out with ReLU
+ * hidden layers and mean-squared-error loss, trained on the GPU by the
+ * library's own forward, backward and Adam kernels.  Parameters are flat:
+ * W1[h1*in], b1[h1], W2[h2*h1], b2[h2], W3[out*h2], b3[out] (row-major,
+ * W[o*I + i]).  X is row-major [n][in], T and Y are [n][out].
+ */
+typedef struct chsga_mlp chsga_mlp;
+int chsga_mlp_create(int32_t n_in, int32_t n_h1, int32_t n_h2, int32_t n_out, uint64_t seed, int device,
+                     chsga_mlp** out);
+void chsga_mlp_destroy(chsga_mlp* m);
+int64_t chsga_mlp_num_params(const chsga_mlp* m);
+int chsga_mlp_get_params(chsga_mlp* m, float* out, int64_t n);
+int chsga_mlp_set_params(chsga_mlp* m, const float* in, int64_t n);
+int chsga_mlp_forward(chsga_mlp* m, const float* X, int64_t n, float* Y);
+/* loss = mean squared error over n*out values; grad[num_params] its gradient */
+int chsga_mlp_loss_grad(chsga_mlp* m, const float* X, const float* T, int64_t n, float* loss, float* grad);
+int chsga_mlp_train(chsga_mlp* m, const float* X, const float* T, int64_t n, int32_t epochs, int32_t batch, float lr,
+                    uint64_t seed, float* final_loss);
 
 #ifdef __cplusplus
 }

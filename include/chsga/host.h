@@ -58,6 +58,10 @@ int chsga_host_load(const char* char_adj, const char* str_adj, const char* ch_pa
 /* Writes <prefix>.char.adj, .str.adj, .ch, .labels, .label_offsets. */
 int chsga_host_save(const chsga_dataset* ds, const char* prefix);
 int chsga_host_save_ch(const chsga_dataset* ds, const char* path);
+/* build_pch: CH of a string graph file; label_offsets may be NULL, which
+ * gives every string vertex length 1 (no vertex weights, as build_pch without
+ * -v, src/CHoverrides/build_pch.cpp:96-114). */
+int chsga_host_build_ch_file(const char* str_adj, const char* label_offsets, const char* out_ch, uint64_t seed);
 int chsga_host_describe(const chsga_dataset* ds, int32_t alphabet, chsga_graph_desc* out);
 int chsga_host_info(const chsga_dataset* ds, chsga_dataset_info* out);
 void chsga_host_free(chsga_dataset* ds);
