@@ -96,6 +96,17 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
         c->h_sin_src[cur[g->str_out_dst[e]]++] = uint32_t(u);
   }
 
+  // compact mode: every char that is not the first of its string has exactly
+  // the in-neighbour p - 1 (the expansion rule of the char graph)
+  c->compact = true;
+  {
+    size_t si = 0;
+    for (int64_t p = 0; p < V && c->compact; ++p) {
+      while (si + 1 < c->h_str_off.size() && c->h_str_off[si + 1] <= uint32_t(p)) ++si;
+      if (int64_t(c->h_str_off[si]) == p) continue;
+      c->compact = (c->h_in_off[p + 1] - c->h_in_off[p] == 1) && (c->h_in_src[c->h_in_off[p]] == uint32_t(p - 1));
+    }
+  }
   std::vector<uint32_t> first_bits((V + 32) / 32, 0);
   for (int64_t s = 0; s < n_s; ++s) first_bits[c->h_str_off[s] >> 5] |= 1u << (c->h_str_off[s] & 31);
 
@@ -220,7 +231,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
 void chsga_ctx::launch_row(const int32_t* H, int32_t* D, uint8_t rc) {
   const hipStream_t s = stream;
   if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
-  hipLaunchKernelGGL(k_row_tile, dim3(unsigned(n_tiles)), dim3(CHSGA_BLOCK), 0, s, H, D, (int32_t*)d_F.p,
+  hipLaunchKernelGGL(compact ? k_row_tile<true> : k_row_tile<false>, dim3(unsigned(n_tiles)), dim3(CHSGA_BLOCK), 0, s, H, D, (int32_t*)d_F.p,
                      (int32_t*)d_Lst.p, (const uint8_t*)d_labels.p, (const uint32_t*)d_in_off.p,
                      (const uint32_t*)d_in_src.p, (const uint32_t*)d_first_bits.p, (const uint32_t*)d_tile_char.p,
                      (const uint32_t*)d_tile_str.p, V, rc, alphabet, msc, ssc, dsc, isc);

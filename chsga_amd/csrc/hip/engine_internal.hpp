@@ -67,6 +67,7 @@ struct chsga_ctx {
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
   // options and counters
   bool profile = false, use_graphs = false, force_staged = false;
+  bool compact = false;  // char graph is the expansion of the string graph
   int64_t launches = 0, rows = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;

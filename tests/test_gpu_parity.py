@@ -136,3 +136,43 @@ def test_c3_full_size_properties():
     assert (p >= 0).all() and (p <= both.lengths()).all()
     assert p[3] == 0 and p[4] == 0
     eng.close()
+
+
+def test_drop_in_drivers(tmp_path):
+    """The ConSGA_* binaries on written input files: the score line equals the
+    oracle's, the traceback driver's verification passes, and flags the
+    reference rejects are rejected."""
+    import subprocess
+    from chsga_amd import WORKLOADS
+    root = Path(__file__).resolve().parent.parent
+    bin_ = root / "chsga_amd" / "bin"
+    w = WORKLOADS["c1"]
+    ds = w.dataset()
+    pre = str(tmp_path / "c1")
+    ds.save(pre)
+    rs = w.readset(ds, n_reads=6)
+    with open(pre + ".reads.fq", "wb") as f:
+        for i in range(len(rs)):
+            f.write(b"@r%d\n%s\n+\n%s\n" % (i, rs[i], b"I" * len(rs[i])))
+    args = ["-I", pre + ".char.adj", "-i", pre + ".str.adj", "-c", pre + ".ch", "-r", pre + ".reads.fq",
+            "-l", pre + ".labels", "-o", pre + ".label_offsets"]
+    ref = O.consga_scores(ds.desc(), rs)
+    out = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    assert lines[0].startswith("input ch time: ") and lines[1].startswith("input time: ")
+    assert lines[2] == "Average times (skipping first read), 95% CI:"
+    assert lines[3].startswith(" Total: ") and "(n=5, df=4)" in lines[3]
+    assert lines[-1] == "score: " + " ".join(str(int(x)) for x in ref)
+    out = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-n", "5"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.stdout.splitlines()[-1] == "score: " + " ".join(str(int(x)) for x in ref[:5])
+    out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-k", "100"], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    assert "[RESULT] ALL CHECKS PASSED" in out.stdout and "(checkpoint every k=100 rows" in out.stdout
+    out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-b"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0 and " Substitutions: NA sec" in out.stdout
+    bad = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-a", "x"], capture_output=True, text=True)
+    assert bad.returncode != 0 and "Usage" in bad.stderr
