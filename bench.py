@@ -36,12 +36,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def row_kernel_bytes(info: dict) -> int:
+def row_kernel_bytes(info: dict, score_bytes: int = 4) -> int:
     """Algorithmic bytes of one k_row_tile launch (one DP row, one read):
-    read H 4V + write D 4V + labels V + in_off 4(V+1) + in_src 4E +
-    first-char bit mask V/8 + F and Lst writes 8 n_s."""
+    read H and write D (2 V scores), labels V, in_off 4(V+1), in_src 4E,
+    first-char bit mask V/8, F and Lst writes (2 n_s scores)."""
     V, E, n_s = info["V"], info["E"], info["n_s"]
-    return 4 * V + 4 * V + V + 4 * (V + 1) + 4 * E + V // 8 + 8 * n_s
+    return 2 * score_bytes * V + V + 4 * (V + 1) + 4 * E + V // 8 + 2 * score_bytes * n_s
+
+
+def batch_kernel_bytes(info: dict, score_bytes: int, slots: int = 64) -> int:
+    """Algorithmic bytes of one k_brow_tile launch (one tick: one DP row of
+    each of `slots` reads).  Per char: its slots' H read and D write; labels
+    and the first-char bit mask once for all slots; per string: the F and Lst
+    writes of every slot, and for its first char the in-CSR entry (8 B), the
+    in-neighbour ids (4 B each) and the in-neighbours' slot rows.  Chars that
+    are not first in their string take their in-neighbour p-1 from the row
+    just read (the char graph is the expansion of the string graph)."""
+    V, n_s, m_str = info["V"], info["n_s"], info["m_str"]
+    B, T = slots, score_bytes
+    return 2 * B * T * V + V + V // 8 + 2 * B * T * n_s + 8 * n_s + 4 * m_str + B * T * m_str
 
 
 def row_bytes_survey(info: dict) -> int:
@@ -161,21 +174,35 @@ def main() -> int:
                            "reads_per_step": args.batch_reads, "slots": args.slots}
         log(f"[bench] batched: {batch_gcups:.2f} GCUPS ({dt:.2f}s)")
 
-    # ---- roofline of the row kernel (profiled, separate short run) -------
+    # ---- roofline of the dominant kernel (profiled, separate short run) ----
+    # HIP events bracket every row-kernel launch on the engine's own stream.
+    score_bytes = 2 if max(reads.lengths()) < 32767 else 4  # int16 storage when provably exact
     eng.set_option(N.OPT_PROFILE, 1)
-    prof = reads.subset([0])
-    eng.align_combo(prof)
-    st = eng.last_stats
+    prof_b = reads.subset(range(min(len(reads), 3 * args.slots)))
+    prof_b = type(prof_b).from_list([prof_b[i][:400] for i in range(len(prof_b))])
+    eng.score_batch(prof_b, slots=args.slots)
+    stb = eng.last_stats
+    eng.align_combo(reads.subset([0]))
+    stp = eng.last_stats
     eng.set_option(N.OPT_PROFILE, 0)
-    avg_ms = st["row_kernel_ms"] / max(1, st["row_kernel_launches"])
-    kb = row_kernel_bytes(info)
-    achieved = kb / (avg_ms * 1e-3) / 1e9
+    b_ms = stb["row_kernel_ms"] / max(1, stb["row_kernel_launches"])
+    p_ms = stp["row_kernel_ms"] / max(1, stp["row_kernel_launches"])
+    kb = batch_kernel_bytes(info, score_bytes, args.slots)
+    pb = row_kernel_bytes(info, score_bytes)
+    achieved = kb / (b_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_row_tile", "kernel_avg_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": kb, "launches_timed": st["row_kernel_launches"],
-                "sweep_avg_us_per_row": round(st["sweep_ms"] * 1e3 / max(1, st["row_kernel_launches"]), 2)}
-    log(f"[bench] row kernel: {avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s")
+                "kernel": "k_brow_tile", "kernel_avg_us": round(b_ms * 1e3, 2),
+                "bytes_per_launch": kb, "launches_timed": stb["row_kernel_launches"],
+                "score_storage_bytes": score_bytes,
+                "plain_kernel": {"kernel": "k_row_tile", "avg_us": round(p_ms * 1e3, 2), "bytes_per_launch": pb,
+                                 "achieved_GBps": round(pb / (p_ms * 1e-3) / 1e9, 1),
+                                 "sweep_post_avg_us_per_row": round(stp["sweep_ms"] * 1e3 /
+                                                                    max(1, stp["row_kernel_launches"]), 2)},
+                "batched_sweep_post_avg_us_per_tick": round(stb["sweep_ms"] * 1e3 /
+                                                            max(1, stb["row_kernel_launches"]), 2)}
+    log(f"[bench] k_brow_tile: {b_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
+        f"k_row_tile: {p_ms * 1e3:.1f} us/launch")
 
     cpu = None
     if not args.no_cpu:
@@ -183,6 +210,7 @@ def main() -> int:
         log(f"[bench] cpu baseline: {cpu['value']} GCUPS on {cpu['cores']} threads, exact={cpu['exact']}")
 
     value = batch_gcups if batch_gcups is not None else plain_gcups
+    model = None
     ms = result.get("batch", result.get("plain"))["ms_per_step"]
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,

@@ -228,13 +228,63 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   hip_check(hipStreamSynchronize(s), "context upload");
 }
 
-void chsga_ctx::launch_row(const int32_t* H, int32_t* D, uint8_t rc) {
+// Row launches for storage type T (int32_t, or int16_t when narrow).
+template <typename T>
+static void launch_row_t(chsga_ctx* c, const void* H, void* D, uint8_t rc) {
+  const hipStream_t s = c->stream;
+  hipLaunchKernelGGL((c->compact ? k_row_tile<true, T> : k_row_tile<false, T>), dim3(unsigned(c->n_tiles)),
+                     dim3(CHSGA_BLOCK), 0, s, (const T*)H, (T*)D, (T*)c->d_F.p, (T*)c->d_Lst.p,
+                     (const uint8_t*)c->d_labels.p, (const uint32_t*)c->d_in_off.p, (const uint32_t*)c->d_in_src.p,
+                     (const uint32_t*)c->d_first_bits.p, (const uint32_t*)c->d_tile_char.p,
+                     (const uint32_t*)c->d_tile_str.p, c->V, rc, c->alphabet, c->msc, c->ssc, c->dsc, c->isc);
+}
+
+template <typename T>
+static void launch_sweeps_t(chsga_ctx* c, int mode) {
+  const hipStream_t s = c->stream;
+  T* F = (T*)c->d_F.p;
+  const T* L = (const T*)c->d_Lst.p;
+  if (mode & 1)
+    for (int32_t j = 1; j < c->j_top; ++j) {
+      const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
+      if (c->h_up_off[k1] == c->h_up_off[k0]) continue;
+      hipLaunchKernelGGL(k_sweep_level<T>, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F,
+                         L, (const uint32_t*)c->d_sw_vert.p, (const uint32_t*)c->d_up_off.p,
+                         (const int2*)c->d_up_in.p, k0, k1, c->isc);
+      c->launches += 1;
+    }
+  if (c->j_top < c->J) {
+    hipLaunchKernelGGL(k_sweep_top<T>, dim3(1), dim3(1024), 0, s, F, L, (const uint32_t*)c->d_sw_vert.p,
+                       (const uint32_t*)c->d_lvl_off.p, c->j_top, c->J, (const uint32_t*)c->d_up_off.p,
+                       (const int2*)c->d_up_in.p, (const uint32_t*)c->d_dn_off.p, (const int2*)c->d_dn_in.p, c->isc,
+                       int32_t(mode));
+    c->launches += 1;
+  }
+  if (mode & 2)
+    for (int32_t j = std::min(c->j_top, c->J) - 1; j >= 0; --j) {
+      const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
+      if (c->h_dn_off[k1] == c->h_dn_off[k0]) continue;
+      hipLaunchKernelGGL(k_sweep_level<T>, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F,
+                         L, (const uint32_t*)c->d_sw_vert.p, (const uint32_t*)c->d_dn_off.p,
+                         (const int2*)c->d_dn_in.p, k0, k1, c->isc);
+      c->launches += 1;
+    }
+}
+
+template <typename T>
+static void launch_post_t(chsga_ctx* c, void* D) {
+  hipLaunchKernelGGL(k_post<T>, dim3(unsigned((c->n_s + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0,
+                     c->stream, (T*)D, (const T*)c->d_F.p, (const uint32_t*)c->d_str_first.p, c->n_s, c->isc);
+  c->launches += 1;
+}
+
+void chsga_ctx::launch_row(const void* H, void* D, uint8_t rc) {
   const hipStream_t s = stream;
   if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
-  hipLaunchKernelGGL(compact ? k_row_tile<true> : k_row_tile<false>, dim3(unsigned(n_tiles)), dim3(CHSGA_BLOCK), 0, s, H, D, (int32_t*)d_F.p,
-                     (int32_t*)d_Lst.p, (const uint8_t*)d_labels.p, (const uint32_t*)d_in_off.p,
-                     (const uint32_t*)d_in_src.p, (const uint32_t*)d_first_bits.p, (const uint32_t*)d_tile_char.p,
-                     (const uint32_t*)d_tile_str.p, V, rc, alphabet, msc, ssc, dsc, isc);
+  if (narrow)
+    launch_row_t<int16_t>(this, H, D, rc);
+  else
+    launch_row_t<int32_t>(this, H, D, rc);
   if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
   launches += 1;
   launch_insertion(D);
@@ -242,53 +292,52 @@ void chsga_ctx::launch_row(const int32_t* H, int32_t* D, uint8_t rc) {
 }
 
 void chsga_ctx::launch_sweeps(int mode) {
-  const hipStream_t s = stream;
-  int32_t* F = (int32_t*)d_F.p;
-  const int32_t* L = (const int32_t*)d_Lst.p;
-  if (mode & 1)
-    for (int32_t j = 1; j < j_top; ++j) {
-      const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
-      if (h_up_off[k1] == h_up_off[k0]) continue;
-      hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F,
-                         L, (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p, k0, k1,
-                         isc);
-      launches += 1;
-    }
-  if (j_top < J) {
-    hipLaunchKernelGGL(k_sweep_top, dim3(1), dim3(1024), 0, s, F, L, (const uint32_t*)d_sw_vert.p,
-                       (const uint32_t*)d_lvl_off.p, j_top, J, (const uint32_t*)d_up_off.p, (const int2*)d_up_in.p,
-                       (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, isc, int32_t(mode));
-    launches += 1;
-  }
-  if (mode & 2)
-    for (int32_t j = std::min(j_top, J) - 1; j >= 0; --j) {
-      const uint32_t k0 = h_lvl_off[j], k1 = h_lvl_off[j + 1];
-      if (h_dn_off[k1] == h_dn_off[k0]) continue;
-      hipLaunchKernelGGL(k_sweep_level, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F,
-                         L, (const uint32_t*)d_sw_vert.p, (const uint32_t*)d_dn_off.p, (const int2*)d_dn_in.p, k0, k1,
-                         isc);
-      launches += 1;
-    }
+  if (narrow)
+    launch_sweeps_t<int16_t>(this, mode);
+  else
+    launch_sweeps_t<int32_t>(this, mode);
 }
 
-void chsga_ctx::launch_insertion(int32_t* D) {
+void chsga_ctx::launch_insertion(void* D) {
   launch_sweeps(3);
-  hipLaunchKernelGGL(k_post, dim3(unsigned((n_s + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0, stream, D,
-                     (const int32_t*)d_F.p, (const uint32_t*)d_str_first.p, n_s, isc);
+  if (narrow)
+    launch_post_t<int16_t>(this, D);
+  else
+    launch_post_t<int32_t>(this, D);
+}
+
+void chsga_ctx::launch_init(void* H, uint8_t c0) {
+  const dim3 g(unsigned((V + CHSGA_BLOCK - 1) / CHSGA_BLOCK));
+  if (narrow)
+    hipLaunchKernelGGL(k_init_row<int16_t>, g, dim3(CHSGA_BLOCK), 0, stream, (int16_t*)H, (const uint8_t*)d_labels.p,
+                       V, c0, alphabet, msc, ssc);
+  else
+    hipLaunchKernelGGL(k_init_row<int32_t>, g, dim3(CHSGA_BLOCK), 0, stream, (int32_t*)H, (const uint8_t*)d_labels.p,
+                       V, c0, alphabet, msc, ssc);
   launches += 1;
 }
 
-void chsga_ctx::launch_init(int32_t* H, uint8_t c0) {
-  hipLaunchKernelGGL(k_init_row, dim3(unsigned((V + CHSGA_BLOCK - 1) / CHSGA_BLOCK)), dim3(CHSGA_BLOCK), 0, stream,
-                     H, (const uint8_t*)d_labels.p, V, c0, alphabet, msc, ssc);
-  launches += 1;
+void chsga_ctx::launch_argmin(const void* D, unsigned long long* key) {
+  if (narrow)
+    hipLaunchKernelGGL(k_argmin<int16_t>, dim3(512), dim3(CHSGA_BLOCK), 0, stream, (const int16_t*)D, V, key);
+  else
+    hipLaunchKernelGGL(k_argmin<int32_t>, dim3(512), dim3(CHSGA_BLOCK), 0, stream, (const int32_t*)D, V, key);
+}
+
+// Largest value any row of a read of length len can store: row 0 is at most
+// max(match, subst) and every later row at most the row above plus del.
+bool chsga_ctx::fits_narrow(int64_t max_len) const {
+  if (!allow_narrow) return false;
+  const int64_t bound = int64_t(std::max(msc, ssc)) + (max_len - 1) * int64_t(dsc);
+  return bound <= 32767;
 }
 
 // Forward pass of one read into the two-row ping-pong buffer; returns the
 // buffer index holding the last row.  ckpt (optional) receives row 0 and
 // every row i with i % k == 0.
 int chsga_ctx::forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows) {
-  int32_t* R[2] = {(int32_t*)d_row[0].p, (int32_t*)d_row[1].p};
+  if (ckpt_rows && narrow) throw std::logic_error("checkpointed forward pass runs with int32 rows");
+  void* R[2] = {d_row[0].p, d_row[1].p};
   int cur = 0;
   launch_init(R[cur], uint8_t(read[0]));
   if (ckpt_rows)
@@ -406,6 +455,9 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         ctx->top_level_opt = value;
         if (value >= 0) ctx->j_top = int32_t(std::min<int64_t>(value, ctx->J));
         break;
+      case CHSGA_OPT_NARROW:
+        ctx->allow_narrow = value != 0;
+        break;
       case CHSGA_OPT_STAGED_WALK:
         ctx->force_staged = value != 0;
         break;
@@ -441,9 +493,10 @@ int chsga_score_reads(chsga_ctx* ctx, const char* const* reads, const int64_t* l
     for (int64_t r = 0; r < n; ++r) {
       const int64_t r0 = now_ns();
       const double pr0 = ctx->prof_row_ms, pi0 = ctx->prof_ins_ms;
+      ctx->narrow = ctx->fits_narrow(lens[r]);
       const int cur = ctx->forward(reads[r], lens[r], 0, nullptr);
-      hipLaunchKernelGGL(k_argmin, dim3(512), dim3(CHSGA_BLOCK), 0, ctx->stream, (const int32_t*)ctx->d_row[cur].p,
-                         ctx->V, dk + r);
+      ctx->launch_argmin(ctx->d_row[cur].p, dk + r);
+      ctx->narrow = false;
       cells += lens[r] * ctx->V;
       if (total_sec || sub_sec || ins_sec || ctx->profile) {
         ctx->flush_profile();
@@ -483,6 +536,7 @@ int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_o
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     const bool prof = ctx->profile;
     ctx->profile = false;
+    ctx->narrow = false;
     const int cur = ctx->forward(read, len, 0, nullptr);
     ctx->profile = prof;
     hip_check(hipMemcpyAsync(row_out, ctx->d_row[cur].p, sizeof(int32_t) * ctx->V, hipMemcpyDeviceToHost,

@@ -68,6 +68,7 @@ struct chsga_ctx {
   // options and counters
   bool profile = false, use_graphs = false, force_staged = false;
   bool compact = false;  // char graph is the expansion of the string graph
+  bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)
   int64_t launches = 0, rows = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -81,9 +82,11 @@ struct chsga_ctx {
   std::shared_ptr<chsga_batch_state> batch;
 
   ~chsga_ctx();
-  void launch_init(int32_t* H, uint8_t c0);
-  void launch_row(const int32_t* H, int32_t* D, uint8_t rc);
-  void launch_insertion(int32_t* D);
+  void launch_init(void* H, uint8_t c0);
+  void launch_row(const void* H, void* D, uint8_t rc);
+  void launch_insertion(void* D);
+  void launch_argmin(const void* D, unsigned long long* key);
+  bool fits_narrow(int64_t max_len) const;
   void launch_sweeps(int mode);  // bit 0: up levels, bit 1: down levels
   void ensure_traceback_tables();
   void ensure_staged_tables(const chsga_graph_desc* g);
@@ -91,6 +94,6 @@ struct chsga_ctx {
                        std::vector<chsga_align_op>& rev, int64_t* v_out);
   int forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows);
   void flush_profile();
-  void launch_tick(int32_t* H, int32_t* D, const void* reads_dev, const int64_t* rd_off_dev,
+  void launch_tick(void* H, void* D, const void* reads_dev, const int64_t* rd_off_dev,
                    const int32_t* order_dev, int32_t n_reads, int32_t* scores_dev, int32_t active_slots);
 };

@@ -176,3 +176,24 @@ def test_drop_in_drivers(tmp_path):
     assert out.returncode == 0 and " Substitutions: NA sec" in out.stdout
     bad = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-a", "x"], capture_output=True, text=True)
     assert bad.returncode != 0 and "Usage" in bad.stderr
+
+
+def test_int16_storage_equals_int32_storage():
+    """Scores stored as int16 (used when every stored value provably stays
+    below 2^15) equal int32 storage and the oracle; a read longer than 32767
+    chars takes the int32 path."""
+    from chsga_amd import native as N
+    ds, rs = small_workload(seed=14, n_reads=40, length=200, n_rate=0.02)
+    ref = O.consga_scores(ds.desc(), rs)
+    eng = Engine(ds)
+    for narrow in (0, 1):
+        eng.set_option(N.OPT_NARROW, narrow)
+        assert np.array_equal(eng.align_combo(rs), ref)
+        assert np.array_equal(eng.score_batch(rs), ref)
+    long_read = ReadSet.generate(ds, seed=5, n_reads=1, length=33000, sub=0.05, ins=0.02, dele=0.02)
+    assert len(long_read[0]) == 33000
+    lref = O.consga_scores(ds.desc(), long_read)
+    assert lref[0] > 0
+    eng.set_option(N.OPT_NARROW, 1)
+    assert np.array_equal(eng.align_combo(long_read), lref)
+    eng.close()
