@@ -153,7 +153,7 @@ struct Builder {
   int64_t compute_score(uint32_t u, SmallMap& m, std::vector<std::pair<int32_t, uint32_t>>& heap,
                         std::vector<int32_t>& dist) const {
     const int64_t din = int64_t(in[u].size()), dout = int64_t(out[u].size());
-    if (din == 0 || dout == 0) return 0;
+    if (din == 0 && dout == 0) return 0;
     const int64_t added = count_shortcuts(u, m, heap, dist);
     const int64_t removed = 1 + din + dout;
     return 1 + 500 * int64_t(level[u]) + (1000 * added) / removed + 10 * removed;
