@@ -54,6 +54,47 @@ using namespace chsga_dev;
 // ---------------------------------------------------------------------------
 // context set-up
 // ---------------------------------------------------------------------------
+// Pull lists of the top set (sweep positions K0 .. n_s-1): up-edges from
+// below the set are external (sources are final when the top kernel starts),
+// the rest are internal and addressed by local index.
+static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<uint32_t>& up_off,
+                      const std::vector<int2>& up_in, const std::vector<uint32_t>& dn_off,
+                      const std::vector<int2>& dn_in, const std::vector<uint32_t>& pos_of_og) {
+  T.j_top = j_top;
+  T.K0 = j_top < c->J ? c->h_lvl_off[j_top] : uint32_t(c->n_s);
+  T.NT = uint32_t(c->n_s) - T.K0;
+  std::vector<uint32_t> ext_off{0}, upi_off{0}, dni_off{0};
+  std::vector<int2> ext, upi, dni;
+  auto local = [&](int2 r) {
+    const uint32_t og = uint32_t(r.x) & 0x7fffffffu;
+    const uint32_t pos = pos_of_og[og];
+    return std::make_pair(pos >= T.K0, make_int2(int32_t((pos - T.K0) | (uint32_t(r.x) & 0x80000000u)), r.y));
+  };
+  for (int64_t k = T.K0; k < c->n_s; ++k) {
+    for (uint32_t e = up_off[k]; e < up_off[k + 1]; ++e) {
+      const auto lr = local(up_in[e]);
+      if (lr.first)
+        upi.push_back(lr.second);
+      else
+        ext.push_back(up_in[e]);
+    }
+    for (uint32_t e = dn_off[k]; e < dn_off[k + 1]; ++e) {
+      const auto lr = local(dn_in[e]);
+      if (!lr.first) throw std::invalid_argument("CH down-edge source below the top set");
+      dni.push_back(lr.second);
+    }
+    ext_off.push_back(uint32_t(ext.size()));
+    upi_off.push_back(uint32_t(upi.size()));
+    dni_off.push_back(uint32_t(dni.size()));
+  }
+  upload(T.ext_off, ext_off, c->stream);
+  upload(T.ext, ext, c->stream);
+  upload(T.upi_off, upi_off, c->stream);
+  upload(T.upi, upi, c->stream);
+  upload(T.dni_off, dni_off, c->stream);
+  upload(T.dni, dni, c->stream);
+}
+
 static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   if (!g || g->V <= 0 || !g->labels || !g->in_off || !g->in_src || g->n_s <= 0 || !g->str_off || !g->rank ||
       !g->layer_off || !g->cc_off || !g->up_off || !g->dn_off)
@@ -107,7 +148,8 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
       c->compact = (c->h_in_off[p + 1] - c->h_in_off[p] == 1) && (c->h_in_src[c->h_in_off[p]] == uint32_t(p - 1));
     }
   }
-  std::vector<uint32_t> first_bits((V + 32) / 32, 0);
+  // two words of padding: kernels read the word after the one holding bit p
+  std::vector<uint32_t> first_bits((V + 32) / 32 + 2, 0);
   for (int64_t s = 0; s < n_s; ++s) first_bits[c->h_str_off[s] >> 5] |= 1u << (c->h_str_off[s] & 31);
 
   // tiles of whole string vertices
@@ -190,15 +232,18 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     }
     dn_off[k + 1] = uint32_t(dn_in.size());
   }
-  // first step handled by the one-workgroup top kernel
-  int32_t jt = J;
-  if (c->top_level_opt >= 0) {
-    jt = std::min<int32_t>(int32_t(c->top_level_opt), J);
-  } else {
-    while (jt > 0 && (c->h_lvl_off[jt] - c->h_lvl_off[jt - 1]) <= 2048 && (n_s - c->h_lvl_off[jt - 1]) <= 16384)
-      --jt;
-  }
-  c->j_top = jt;
+  // first step of the LDS-resident top kernel: as many top levels as fit
+  // kTopPlain (plain) / kTopBatch (batched) string vertices
+  auto top_step = [&](int64_t max_nt) {
+    int32_t jt = J;
+    while (jt > 0 && n_s - int64_t(c->h_lvl_off[jt - 1]) <= max_nt) --jt;
+    return jt;
+  };
+  c->j_top = top_step(kTopPlain);
+  if (c->top_level_opt >= 0) c->j_top = std::max(c->j_top, std::min<int32_t>(int32_t(c->top_level_opt), J));
+  const int32_t j_top_b = top_step(kTopBatch);
+  std::vector<uint32_t> pos_of_og(n_s);
+  for (int64_t og = 0; og < n_s; ++og) pos_of_og[og] = kpos[g->rank[og]];
 
   // every check above is host-only; the device is touched from here on
   hip_check(hipSetDevice(c->device), "hipSetDevice");
@@ -219,6 +264,8 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_dn_off, dn_off, s);
   upload(c->d_dn_in, dn_in, s);
   c->ensure_staged_tables(g);
+  build_top(c, c->top_plain, c->j_top, up_off, up_in, dn_off, dn_in, pos_of_og);
+  build_top(c, c->top_batch, j_top_b, up_off, up_in, dn_off, dn_in, pos_of_og);
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);
   c->d_F.alloc(sizeof(int32_t) * n_s);
@@ -232,7 +279,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
 template <typename T>
 static void launch_row_t(chsga_ctx* c, const void* H, void* D, uint8_t rc) {
   const hipStream_t s = c->stream;
-  hipLaunchKernelGGL((c->compact ? k_row_tile<true, T> : k_row_tile<false, T>), dim3(unsigned(c->n_tiles)),
+  hipLaunchKernelGGL((c->compact ? k_row_tile8<true, T> : k_row_tile8<false, T>), dim3(unsigned(c->n_tiles)),
                      dim3(CHSGA_BLOCK), 0, s, (const T*)H, (T*)D, (T*)c->d_F.p, (T*)c->d_Lst.p,
                      (const uint8_t*)c->d_labels.p, (const uint32_t*)c->d_in_off.p, (const uint32_t*)c->d_in_src.p,
                      (const uint32_t*)c->d_first_bits.p, (const uint32_t*)c->d_tile_char.p,
@@ -253,10 +300,12 @@ static void launch_sweeps_t(chsga_ctx* c, int mode) {
                          (const int2*)c->d_up_in.p, k0, k1, c->isc);
       c->launches += 1;
     }
-  if (c->j_top < c->J) {
-    hipLaunchKernelGGL(k_sweep_top<T>, dim3(1), dim3(1024), 0, s, F, L, (const uint32_t*)c->d_sw_vert.p,
-                       (const uint32_t*)c->d_lvl_off.p, c->j_top, c->J, (const uint32_t*)c->d_up_off.p,
-                       (const int2*)c->d_up_in.p, (const uint32_t*)c->d_dn_off.p, (const int2*)c->d_dn_in.p, c->isc,
+  const TopSet& TS = c->top_plain;
+  if (TS.NT > 0) {
+    hipLaunchKernelGGL(k_sweep_top_lds<T>, dim3(1), dim3(1024), size_t(2) * TS.NT * sizeof(int32_t), s, F, L,
+                       (const uint32_t*)c->d_sw_vert.p, TS.K0, TS.NT, (const uint32_t*)c->d_lvl_off.p, TS.j_top, c->J,
+                       (const uint32_t*)TS.ext_off.p, (const int2*)TS.ext.p, (const uint32_t*)TS.upi_off.p,
+                       (const int2*)TS.upi.p, (const uint32_t*)TS.dni_off.p, (const int2*)TS.dni.p, c->isc,
                        int32_t(mode));
     c->launches += 1;
   }
@@ -451,18 +500,12 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
       case CHSGA_OPT_USE_GRAPHS:
         ctx->use_graphs = value != 0;
         break;
-      case CHSGA_OPT_TOP_LEVEL:
-        ctx->top_level_opt = value;
-        if (value >= 0) ctx->j_top = int32_t(std::min<int64_t>(value, ctx->J));
-        break;
       case CHSGA_OPT_NARROW:
         ctx->allow_narrow = value != 0;
         break;
       case CHSGA_OPT_STAGED_WALK:
         ctx->force_staged = value != 0;
         break;
-      case CHSGA_OPT_TILE_CHARS:
-        throw std::invalid_argument("tile size is fixed at context creation");
       default:
         throw std::invalid_argument("unknown option");
     }

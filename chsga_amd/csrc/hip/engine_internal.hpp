@@ -36,6 +36,15 @@ struct DevMem {
 
 struct chsga_batch_state;
 
+// Top levels of the CH handled by one LDS-resident workgroup.
+constexpr int64_t kTopPlain = 4096;  // string vertices: 2 x 4 B each in LDS
+constexpr int64_t kTopBatch = 128;   // string vertices: 2 x 64 slots x 4 B each in LDS
+struct TopSet {
+  int32_t j_top = 0;
+  uint32_t K0 = 0, NT = 0;
+  chsga_engine::DevMem ext_off, ext, upi_off, upi, dni_off, dni;
+};
+
 struct OgEdge {
   uint32_t v;
   int32_t w;
@@ -51,7 +60,7 @@ struct chsga_ctx {
   int64_t n_tiles = 0;
   int32_t J = 0, j_top = 0;
   int64_t top_level_opt = -1;
-  int64_t tile_chars = 2048;
+  int64_t tile_chars = 2040;  // one 256 x 8 block round even when c0 is not 8-aligned
   int64_t batch_tile_chars = 512;
   std::vector<uint32_t> h_lvl_off, h_up_off, h_dn_off;
   // host copies for traceback
@@ -63,6 +72,7 @@ struct chsga_ctx {
   // device graph
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
+  TopSet top_plain, top_batch;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
   // options and counters
@@ -90,7 +100,7 @@ struct chsga_ctx {
   void launch_sweeps(int mode);  // bit 0: up levels, bit 1: down levels
   void ensure_traceback_tables();
   void ensure_staged_tables(const chsga_graph_desc* g);
-  void staged_walk_row(const int32_t* prow_dev, const char* read, int64_t i, int64_t v0,
+  void staged_walk_row(const int32_t* prow_dev, const int32_t* fin_dev, const char* read, int64_t i, int64_t v0,
                        std::vector<chsga_align_op>& rev, int64_t* v_out);
   int forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows);
   void flush_profile();

@@ -59,12 +59,14 @@ def read_features(reads, ds_info: dict, k) -> np.ndarray:
 
 def traceback_peak_bytes(length: int, V: int, k: int) -> int:
     """Device bytes the checkpointed traceback allocates: the checkpoint rows
-    (rows 0, k, 2k, ...), the segment store (min(k, len-1) + 1 rows) and the
-    walk buffer; equals chsga_stats.peak_device_bytes."""
+    (rows 0, k, 2k, ...), the segment store (min(k, len-1) + 1 rows; rows are
+    padded to a multiple of 64 scores) and the walk buffer; equals
+    chsga_stats.peak_device_bytes."""
     n_ckpt = (length - 1) // k + 1
     seg_rows = min(k, length - 1) + 1
     walk = 16 * (2 * seg_rows + 1024)
-    return 4 * V * (n_ckpt + seg_rows) + walk
+    row_stride = (V + 63) // 64 * 64
+    return 4 * row_stride * (n_ckpt + seg_rows) + walk
 
 
 class CostModel:

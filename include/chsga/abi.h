@@ -126,8 +126,6 @@ typedef struct chsga_stats {
 /* Context options, chsga_ctx_set_option. */
 #define CHSGA_OPT_PROFILE 1      /* 1: time row kernels with HIP events      */
 #define CHSGA_OPT_USE_GRAPHS 2   /* 1: replay captured hipGraphs per rows     */
-#define CHSGA_OPT_TOP_LEVEL 3    /* first CH level handled by the one-group top kernel (-1 auto) */
-#define CHSGA_OPT_TILE_CHARS 4   /* target chars per row-kernel tile (0 auto) */
 #define CHSGA_OPT_STAGED_WALK 5  /* 1: walk every traceback row through its stages (validation) */
 #define CHSGA_OPT_NARROW 6       /* 1 (default): store scores as int16 when provably < 2^15; 0: int32 */
 

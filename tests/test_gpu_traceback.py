@@ -61,10 +61,10 @@ def test_staged_walk_gives_valid_paths(tmp_path):
         for _ in range(3):
             r = R.rbytes(rng, int(rng.integers(2, 30)))
             tb = eng.align_read_with_traceback(r, k=4)
+            assert eng.last_stats["walk_fallbacks"] == len(r) - 1
             assert tb.score == eng.align_combo(ReadSet.from_list([r]))[0]
             rc, bad = O.validate_path(d, r, tb.ops, tb.score)
             assert rc == 0, (t, r, rc, bad)
-            assert eng.last_stats["walk_fallbacks"] == len(r) - 1
             checked += 1
         eng.close()
     assert checked == 18

@@ -32,7 +32,7 @@ def test_numpy_reference_gradient_matches_finite_differences():
 
 def test_traceback_bytes_formula():
     # rows 0, k, 2k, ... plus min(k, len-1)+1 segment rows plus the walk buffer
-    assert L.traceback_peak_bytes(1001, 10, 500) == 4 * 10 * (3 + 501) + 16 * (2 * 501 + 1024)
+    assert L.traceback_peak_bytes(1001, 10, 500) == 4 * 64 * (3 + 501) + 16 * (2 * 501 + 1024)
 
 
 def test_lpt_packing_beats_fifo_on_a_lognormal_stream():
