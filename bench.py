@@ -91,11 +91,11 @@ def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--batch-reads", type=int, default=256, help="reads per batched step")
-    ap.add_argument("--plain-reads", type=int, default=4, help="reads per plain step")
+    ap.add_argument("--batch-reads", type=int, default=384, help="reads per batched step")
+    ap.add_argument("--plain-reads", type=int, default=2, help="reads per plain step")
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -144,6 +144,7 @@ def main() -> int:
     if args.mode in ("both", "plain"):
         for w in range(args.warmup):
             eng.align_combo(step_reads(w, args.plain_reads))
+            log(f"[bench] plain warm-up {w} done")
         cells = 0
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -157,17 +158,24 @@ def main() -> int:
         log(f"[bench] plain: {plain_gcups:.2f} GCUPS ({dt:.2f}s, {args.steps} steps)")
 
     # ---- batched path -------------------------------------------------------
+    # A step is one stream of batch_reads reads through the 64 slots; the
+    # warm-up is a short batch that allocates the slot buffers.
     batch_gcups = None
     if args.mode in ("both", "batch"):
         for w in range(args.warmup):
             rs = step_reads(w, args.batch_reads)
+            rs = type(rs).from_list([rs[i][:64] for i in range(min(len(rs), 2 * args.slots))])
             eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds))
+            log(f"[bench] batched warm-up {w} done")
         cells = 0
         t0 = time.perf_counter()
         for k in range(args.steps):
             rs = step_reads(args.warmup + k, args.batch_reads)
-            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds))
+            ts = time.perf_counter()
+            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds), use_graphs=True)
             cells += int(rs.lengths().sum()) * V
+            log(f"[bench] batched step {k}: {len(rs)} reads, {eng.last_stats['rows']} ticks, "
+                f"{time.perf_counter() - ts:.1f}s")
         dt = time.perf_counter() - t0
         batch_gcups = cells / dt / 1e9
         result["batch"] = {"gcups": round(batch_gcups, 3), "ms_per_step": round(dt * 1e3 / args.steps, 2),
