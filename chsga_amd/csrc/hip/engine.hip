@@ -254,6 +254,21 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_in_off, c->h_in_off, s);
   upload(c->d_in_src, c->h_in_src, s);
   upload(c->d_first_bits, first_bits, s);
+  if (c->compact) {
+    // up to two in-neighbour chars per first char; .x = -2 when there are more
+    std::vector<int2> nb2(V, make_int2(-1, -1));
+    for (int64_t st = 0; st < n_s; ++st) {
+      const uint32_t p = c->h_str_off[st];
+      const uint32_t e0 = c->h_in_off[p], e1 = c->h_in_off[p + 1];
+      if (e1 - e0 > 2) {
+        nb2[p] = make_int2(-2, -1);
+      } else {
+        if (e1 > e0) nb2[p].x = int32_t(c->h_in_src[e0]);
+        if (e1 > e0 + 1) nb2[p].y = int32_t(c->h_in_src[e0 + 1]);
+      }
+    }
+    upload(c->d_nb2, nb2, s);
+  }
   upload(c->d_tile_char, tile_char, s);
   upload(c->d_tile_str, tile_str, s);
   upload(c->d_str_first, c->h_str_off, s);

@@ -71,6 +71,7 @@ struct chsga_ctx {
   chsga_engine::DevMem d_sin_off, d_sin_src;
   // device graph
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
+  chsga_engine::DevMem d_nb2;  // compact mode: <= 2 in-neighbours of each first char
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
   TopSet top_plain, top_batch;
   // working rows

@@ -1,0 +1,46 @@
+"""A short fixed workload for profilers (rocprofv3 --kernel-trace / --pmc).
+
+    python -m chsga_amd.profile_run [--workload c3] [--rows 300] [--reads 64] [--mode both|plain|batch]
+
+Takes the first `rows` chars of `reads` reads of the workload and runs read 0
+on the plain path (rows - 1 DP rows) and all of them on the batched path
+(`rows` ticks of 64 slots), printing one line per stage.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+from pathlib import Path
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+
+
+def main() -> int:
+    from chsga_amd import WORKLOADS, Engine, ReadSet
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--rows", type=int, default=300)
+    ap.add_argument("--reads", type=int, default=64)
+    ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
+    a = ap.parse_args()
+    w = WORKLOADS[a.workload]
+    ds = w.dataset()
+    rs = w.readset(ds, n_reads=a.reads)
+    short = ReadSet.from_list([rs[i][: a.rows] for i in range(len(rs))])
+    eng = Engine(ds, w.alphabet)
+    print(f"[profile_run] {a.workload}: {ds.info()}", flush=True)
+    if a.mode in ("both", "plain"):
+        t = time.time()
+        s = eng.align_combo(short.subset([0]))
+        print(f"[profile_run] plain: 1 read x {len(short[0])} rows in {time.time() - t:.2f}s, score {s[0]}", flush=True)
+    if a.mode in ("both", "batch"):
+        t = time.time()
+        s = eng.score_batch(short)
+        print(f"[profile_run] batched: {len(short)} reads x {a.rows} rows in {time.time() - t:.2f}s", flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
