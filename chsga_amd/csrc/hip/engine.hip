@@ -38,7 +38,10 @@ void DevMem::release() {
 template <class T>
 static void upload(DevMem& d, const std::vector<T>& h, hipStream_t s) {
   d.alloc(std::max<size_t>(1, h.size()) * sizeof(T));
-  if (!h.empty()) hip_check(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  if (h.empty()) return;
+  hip_check(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  // the source is pageable and may be a temporary: wait until it has been read
+  hip_check(hipStreamSynchronize(s), "upload");
 }
 
 static int64_t now_ns() {
@@ -57,9 +60,11 @@ using namespace chsga_dev;
 // Pull lists of the top set (sweep positions K0 .. n_s-1): up-edges from
 // below the set are external (sources are final when the top kernel starts),
 // the rest are internal and addressed by local index.
+// With by_pos, external pulls name their source by sweep position (the
+// batched path's F/Lst layout), else by original string id.
 static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<uint32_t>& up_off,
                       const std::vector<int2>& up_in, const std::vector<uint32_t>& dn_off,
-                      const std::vector<int2>& dn_in, const std::vector<uint32_t>& pos_of_og) {
+                      const std::vector<int2>& dn_in, const std::vector<uint32_t>& pos_of_og, bool by_pos) {
   T.j_top = j_top;
   T.K0 = j_top < c->J ? c->h_lvl_off[j_top] : uint32_t(c->n_s);
   T.NT = uint32_t(c->n_s) - T.K0;
@@ -75,6 +80,10 @@ static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<
       const auto lr = local(up_in[e]);
       if (lr.first)
         upi.push_back(lr.second);
+      else if (by_pos)
+        ext.push_back(make_int2(int32_t(pos_of_og[uint32_t(up_in[e].x) & 0x7fffffffu] |
+                                        (uint32_t(up_in[e].x) & 0x80000000u)),
+                                up_in[e].y));
       else
         ext.push_back(up_in[e]);
     }
@@ -257,11 +266,13 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   if (c->compact) {
     // up to two in-neighbour chars per first char; .x = -2 when there are more
     std::vector<int2> nb2(V, make_int2(-1, -1));
+    c->nb2_all = true;
     for (int64_t st = 0; st < n_s; ++st) {
       const uint32_t p = c->h_str_off[st];
       const uint32_t e0 = c->h_in_off[p], e1 = c->h_in_off[p + 1];
       if (e1 - e0 > 2) {
         nb2[p] = make_int2(-2, -1);
+        c->nb2_all = false;
       } else {
         if (e1 > e0) nb2[p].x = int32_t(c->h_in_src[e0]);
         if (e1 > e0 + 1) nb2[p].y = int32_t(c->h_in_src[e0 + 1]);
@@ -274,13 +285,41 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_str_first, c->h_str_off, s);
   upload(c->d_sw_vert, sw_vert, s);
   upload(c->d_lvl_off, c->h_lvl_off, s);
-  upload(c->d_up_off, up_off, s);
+  {
+    // the batched level sweep reads up to kSweepPad offsets past its level
+    std::vector<uint32_t> pad(up_off);
+    pad.resize(n_s + 1 + kSweepPad, up_off[n_s]);
+    upload(c->d_up_off, pad, s);
+  }
   upload(c->d_up_in, up_in, s);
-  upload(c->d_dn_off, dn_off, s);
+  {
+    std::vector<uint32_t> pad(dn_off);
+    pad.resize(n_s + 1 + kSweepPad, dn_off[n_s]);
+    upload(c->d_dn_off, pad, s);
+  }
   upload(c->d_dn_in, dn_in, s);
   c->ensure_staged_tables(g);
-  build_top(c, c->top_plain, c->j_top, up_off, up_in, dn_off, dn_in, pos_of_og);
-  build_top(c, c->top_batch, j_top_b, up_off, up_in, dn_off, dn_in, pos_of_og);
+  build_top(c, c->top_plain, c->j_top, up_off, up_in, dn_off, dn_in, pos_of_og, false);
+  build_top(c, c->top_batch, j_top_b, up_off, up_in, dn_off, dn_in, pos_of_og, true);
+  {
+    auto by_pos = [&](const std::vector<int2>& in) {
+      std::vector<int2> out(in.size() + 2, make_int2(0, 0));  // 2 readable entries past the end
+      for (size_t e = 0; e < in.size(); ++e)
+        out[e] = make_int2(int32_t(pos_of_og[uint32_t(in[e].x) & 0x7fffffffu] | (uint32_t(in[e].x) & 0x80000000u)),
+                           in[e].y);
+      return out;
+    };
+    upload(c->d_bup_in, by_pos(up_in), s);
+    upload(c->d_bdn_in, by_pos(dn_in), s);
+    std::vector<uint2> span(n_s + kSweepPad, make_uint2(0, 0));
+    for (int64_t k = 0; k < n_s; ++k) {
+      const uint32_t og = sw_vert[k];
+      const bool up_pulls = up_off[k + 1] > up_off[k];
+      span[k] = make_uint2(c->h_str_off[og] | (up_pulls ? 0x80000000u : 0u), c->h_str_off[og + 1]);
+    }
+    upload(c->d_bspan, span, s);
+    upload(c->d_bpos, pos_of_og, s);
+  }
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);
   c->d_F.alloc(sizeof(int32_t) * n_s);

@@ -38,6 +38,8 @@ struct chsga_batch_state;
 
 // Top levels of the CH handled by one LDS-resident workgroup.
 constexpr int64_t kTopPlain = 4096;  // string vertices: 2 x 4 B each in LDS
+// entries the batched level sweep may read past a table's end (>= its vertices per wave)
+constexpr int64_t kSweepPad = 8;
 constexpr int64_t kTopBatch = 128;   // string vertices: 2 x 64 slots x 4 B each in LDS
 struct TopSet {
   int32_t j_top = 0;
@@ -72,7 +74,14 @@ struct chsga_ctx {
   // device graph
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
   chsga_engine::DevMem d_nb2;  // compact mode: <= 2 in-neighbours of each first char
+  bool nb2_all = false;  // compact, and no first char has more than 2 in-neighbours
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
+  // batched path: F/Lst rows are stored by sweep position k, so a CH level is
+  // one contiguous range.  bpos[s] = k of string s; bup_in/bdn_in = the pull
+  // lists with positions in place of string ids; bspan[k] = {first char of
+  // the string | 1 << 31 when up-pulls may lower F below the row's first
+  // value, end char}
+  chsga_engine::DevMem d_bpos, d_bup_in, d_bdn_in, d_bspan;
   TopSet top_plain, top_batch;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
