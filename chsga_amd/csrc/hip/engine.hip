@@ -102,6 +102,15 @@ static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<
   upload(T.upi, upi, c->stream);
   upload(T.dni_off, dni_off, c->stream);
   upload(T.dni, dni, c->stream);
+  T.n_upi = uint32_t(upi.size());
+  T.n_dni = uint32_t(dni.size());
+}
+
+// LDS bytes of k_bsweep_top_lds for a top set: F and Lst values of every slot,
+// then both internal pull lists with their offsets
+static size_t batch_top_lds_bytes(const TopSet& T) {
+  return size_t(2) * T.NT * 64 * sizeof(int32_t) + size_t(2) * (T.NT + 1) * sizeof(uint32_t) +
+         size_t(T.n_upi + T.n_dni) * sizeof(int2);
 }
 
 static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
@@ -279,6 +288,21 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
       }
     }
     upload(c->d_nb2, nb2, s);
+    // label classes for the batched compact kernel (see read_class)
+    std::vector<uint8_t> lclass(V);
+    bool acgtn = true;
+    for (int64_t p = 0; p < V; ++p) {
+      switch (g->labels[p]) {
+        case 'A': lclass[p] = 1; break;
+        case 'C': lclass[p] = 2; break;
+        case 'G': lclass[p] = 4; break;
+        case 'T': lclass[p] = 8; break;
+        case 'N': lclass[p] = c->alphabet == CHSGA_ALPHABET_MATCH_N ? 63 : 16; break;
+        default: acgtn = false;
+      }
+    }
+    c->batch_fast = c->nb2_all && acgtn;
+    if (c->batch_fast) upload(c->d_lclass, lclass, s);
   }
   upload(c->d_tile_char, tile_char, s);
   upload(c->d_tile_str, tile_str, s);
@@ -300,7 +324,19 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_dn_in, dn_in, s);
   c->ensure_staged_tables(g);
   build_top(c, c->top_plain, c->j_top, up_off, up_in, dn_off, dn_in, pos_of_og, false);
-  build_top(c, c->top_batch, j_top_b, up_off, up_in, dn_off, dn_in, pos_of_og, true);
+  {
+    // the batched top set also keeps its pull lists in LDS: shrink it until
+    // one workgroup's LDS holds everything
+    int lds_max = 0;
+    hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device), "attribute");
+    int32_t jt = j_top_b;
+    for (;;) {
+      build_top(c, c->top_batch, jt, up_off, up_in, dn_off, dn_in, pos_of_og, true);
+      if (c->top_batch.NT == 0 || batch_top_lds_bytes(c->top_batch) <= size_t(lds_max)) break;
+      ++jt;
+    }
+    c->top_batch_lds = batch_top_lds_bytes(c->top_batch);
+  }
   {
     auto by_pos = [&](const std::vector<int2>& in) {
       std::vector<int2> out(in.size() + 2, make_int2(0, 0));  // 2 readable entries past the end
@@ -459,8 +495,9 @@ int chsga_ctx::forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_
 
 void chsga_ctx::flush_profile() {
   if (!profile || ev_used == 0) return;
-  hip_check(hipEventSynchronize(ev_pool[ev_used - 1]), "event sync");
   for (size_t i = 0; i + 2 < ev_used; i += 3) {
+    // each triple is on one stream; concurrent batch groups use several
+    hip_check(hipEventSynchronize(ev_pool[i + 2]), "event sync");
     float a = 0, b = 0;
     hip_check(hipEventElapsedTime(&a, ev_pool[i], ev_pool[i + 1]), "elapsed");
     hip_check(hipEventElapsedTime(&b, ev_pool[i + 1], ev_pool[i + 2]), "elapsed");

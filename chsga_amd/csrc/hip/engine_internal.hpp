@@ -40,10 +40,13 @@ struct chsga_batch_state;
 constexpr int64_t kTopPlain = 4096;  // string vertices: 2 x 4 B each in LDS
 // entries the batched level sweep may read past a table's end (>= its vertices per wave)
 constexpr int64_t kSweepPad = 8;
-constexpr int64_t kTopBatch = 128;   // string vertices: 2 x 64 slots x 4 B each in LDS
+// chars the batched row buffers hold past V (>= one row-pass chunk)
+constexpr int64_t kRowPad = 32;
+constexpr int64_t kTopBatch = 0;     // string vertices: 2 x 64 slots x 4 B each in LDS
 struct TopSet {
   int32_t j_top = 0;
   uint32_t K0 = 0, NT = 0;
+  uint32_t n_upi = 0, n_dni = 0;  // internal pull counts
   chsga_engine::DevMem ext_off, ext, upi_off, upi, dni_off, dni;
 };
 
@@ -75,6 +78,8 @@ struct chsga_ctx {
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
   chsga_engine::DevMem d_nb2;  // compact mode: <= 2 in-neighbours of each first char
   bool nb2_all = false;  // compact, and no first char has more than 2 in-neighbours
+  bool batch_fast = false;  // nb2_all and every label in ACGTN: the batched compact row kernel
+  chsga_engine::DevMem d_lclass;  // match class of each char's label (batched compact kernel)
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
   // batched path: F/Lst rows are stored by sweep position k, so a CH level is
   // one contiguous range.  bpos[s] = k of string s; bup_in/bdn_in = the pull
@@ -83,6 +88,7 @@ struct chsga_ctx {
   // value, end char}
   chsga_engine::DevMem d_bpos, d_bup_in, d_bdn_in, d_bspan;
   TopSet top_plain, top_batch;
+  size_t top_batch_lds = 0;  // dynamic LDS bytes of the batched top kernel
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
   // options and counters
@@ -114,6 +120,4 @@ struct chsga_ctx {
                        std::vector<chsga_align_op>& rev, int64_t* v_out);
   int forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_rows);
   void flush_profile();
-  void launch_tick(void* H, void* D, const void* reads_dev, const int64_t* rd_off_dev,
-                   const int32_t* order_dev, int32_t n_reads, int32_t* scores_dev, int32_t active_slots);
 };

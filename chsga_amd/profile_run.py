@@ -1,10 +1,10 @@
 """A short fixed workload for profilers (rocprofv3 --kernel-trace / --pmc).
 
-    python -m chsga_amd.profile_run [--workload c3] [--rows 300] [--reads 64] [--mode both|plain|batch]
+    python -m chsga_amd.profile_run [--workload c3] [--rows 300] [--reads 64] [--slots 64] [--mode both|plain|batch]
 
 Takes the first `rows` chars of `reads` reads of the workload and runs read 0
 on the plain path (rows - 1 DP rows) and all of them on the batched path
-(`rows` ticks of 64 slots), printing one line per stage.
+(`rows` ticks of `slots` slots, in groups of 64), printing one line per stage.
 """
 from __future__ import annotations
 
@@ -22,6 +22,7 @@ def main() -> int:
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--rows", type=int, default=300)
     ap.add_argument("--reads", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
     a = ap.parse_args()
     w = WORKLOADS[a.workload]
@@ -36,7 +37,7 @@ def main() -> int:
         print(f"[profile_run] plain: 1 read x {len(short[0])} rows in {time.time() - t:.2f}s, score {s[0]}", flush=True)
     if a.mode in ("both", "batch"):
         t = time.time()
-        s = eng.score_batch(short)
+        s = eng.score_batch(short, slots=a.slots)
         print(f"[profile_run] batched: {len(short)} reads x {a.rows} rows in {time.time() - t:.2f}s", flush=True)
     eng.close()
     return 0

@@ -65,6 +65,38 @@ def test_batch_calls_back_to_back_and_async():
     eng.close()
 
 
+@pytest.mark.parametrize("slots", [128, 96, 200])
+def test_slot_groups_match_oracle(slots):
+    """More than 64 slots run as groups of 64 on separate streams, each with
+    its share of the queue; scores stay exact with and without graphs."""
+    ds, rs = small_workload(seed=23, n_reads=150, length=70, error=0.05, n_rate=0.02)
+    ref = O.consga_scores(ds.desc(), rs)
+    eng = Engine(ds)
+    _say("groups", slots)
+    assert np.array_equal(eng.score_batch(rs, slots=slots), ref)
+    assert np.array_equal(eng.score_batch(rs, slots=slots, order=plan_order(rs, ds), use_graphs=True), ref)
+    assert np.array_equal(eng.score_batch(rs.subset(range(7)), slots=slots), ref[:7])
+    eng.close()
+
+
+def test_first_chars_with_three_in_neighbours(tmp_path):
+    """A string with three predecessors takes the general row kernel of the
+    batched path; its scores equal the oracle's."""
+    rng = np.random.default_rng(9)
+    strings = [R.rbytes(rng, int(rng.integers(1, 6))) for _ in range(12)]
+    edges = [(0, 3), (1, 3), (2, 3), (3, 4), (3, 5), (4, 6), (5, 6), (6, 7), (7, 8), (8, 9), (2, 9), (9, 10),
+             (10, 11), (4, 11)]
+    p = R.write_dataset(tmp_path, strings, edges, "tri")
+    ds = Dataset.load(p["char.adj"], p["str.adj"], None, p["labels"], p["label_offsets"])
+    reads = ReadSet.from_list([R.rbytes(rng, int(rng.integers(1, 30))) for _ in range(70)])
+    ref = O.consga_scores(ds.desc(), reads)
+    eng = Engine(ds)
+    assert np.array_equal(eng.score_batch(reads), ref)
+    assert np.array_equal(eng.score_batch(reads, slots=128, use_graphs=True), ref)
+    assert np.array_equal(eng.align_combo(reads), ref)
+    eng.close()
+
+
 def test_multi_component_and_long_strings(tmp_path):
     rng = np.random.default_rng(4)
     strings, edges = [], []
