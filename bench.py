@@ -9,8 +9,12 @@ log-normal median 3 kbp clipped to 1-30 kbp, 2% error, 1% N, mismatchN).
              job, inputs resident in HBM;
   plain      GCUPS of the plain path (one read per launch sequence) on the
              same graph;
-  roofline   the row kernel (k_row_tile): algorithmic bytes per launch over
-             its mean duration from HIP events on the engine's stream;
+  roofline   the batched row kernel (k_brow_tile, one DP row of 64 reads per
+             launch): algorithmic bytes per launch over its mean duration from
+             HIP events on the engine's stream; traffic = HBM bytes per launch
+             from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/
+             (scaled by a plain copy of known size), when that summary was
+             taken on the same per-launch bytes;
   cpu_baseline  the oracle (C++/OpenMP restatement of ConSGA) on a bounded
              sample of the same workload on this host's cores.
 """
@@ -55,6 +59,22 @@ def batch_kernel_bytes(info: dict, score_bytes: int, slots: int = 64) -> int:
     V, n_s, m_str = info["V"], info["n_s"], info["m_str"]
     B, T = slots, score_bytes
     return 2 * B * T * V + V + V // 8 + 2 * B * T * n_s + 8 * n_s + 4 * m_str + B * T * m_str
+
+
+def measured_traffic(bytes_per_launch: int):
+    """HBM bytes per k_brow_tile launch from the committed counter summary,
+    or None when it was taken on a different workload."""
+    p = ROOT / "profiles" / "k_brow_tile_c3_hbm_traffic.md"
+    if not p.exists():
+        return None
+    vals = {}
+    for line in p.read_text().splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch") and v.strip().isdigit():
+            vals[k.strip()] = int(v.strip())
+    if vals.get("algorithmic_bytes_per_launch") != bytes_per_launch:
+        return None
+    return vals.get("traffic_bytes_per_launch")
 
 
 def row_bytes_survey(info: dict) -> int:
@@ -199,7 +219,7 @@ def main() -> int:
     pb = row_kernel_bytes(info, score_bytes)
     achieved = kb / (b_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kb),
                 "kernel": "k_brow_tile", "kernel_avg_us": round(b_ms * 1e3, 2),
                 "bytes_per_launch": kb, "launches_timed": stb["row_kernel_launches"],
                 "score_storage_bytes": score_bytes,

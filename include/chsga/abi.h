@@ -147,7 +147,10 @@ int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_o
 
 /* Batched score-only alignment: reads share slot-interleaved rows on the GPU
  * and a device work queue refills a slot when its read ends.  order[n] (may
- * be NULL) is the order in which reads enter the queue.  slots in {16,32,64}. */
+ * be NULL) is the order in which reads enter the queue.  slots in 1..256
+ * (default 64): every 64 slots form a group with its own rows and queue,
+ * running on its own stream; the order is dealt to the groups by least
+ * queued work. */
 typedef struct chsga_batch_opts {
   int32_t slots;
   int32_t use_graphs;
