@@ -140,6 +140,7 @@ def main() -> int:
         return 0
 
     from chsga_amd import WORKLOADS, Engine
+    from chsga_amd import learned as L
     from chsga_amd import native as N
     from chsga_amd.scheduler import plan_order
 
@@ -153,6 +154,11 @@ def main() -> int:
     reads = wl.readset(ds, n_reads=n_needed)
     eng = Engine(ds, wl.alphabet)
     eng.set_option(N.OPT_PROFILE, 0)
+    # queue order: longest predicted time first, predictions from the learned
+    # cost model (trained on the GPU, chsga_amd/data) or the plain formula
+    model = L.load_default()
+    scheduler = "learned cost model (LPT)" if model is not None else "plain formula rows x V (LPT)"
+    log(f"[bench] scheduler: {scheduler}")
     V = info["V"]
     result = {}
 
@@ -185,14 +191,14 @@ def main() -> int:
         for w in range(args.warmup):
             rs = step_reads(w, args.batch_reads)
             rs = type(rs).from_list([rs[i][:64] for i in range(min(len(rs), 2 * args.slots))])
-            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds))
+            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds, model))
             log(f"[bench] batched warm-up {w} done")
         cells = 0
         t0 = time.perf_counter()
         for k in range(args.steps):
             rs = step_reads(args.warmup + k, args.batch_reads)
             ts = time.perf_counter()
-            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds), use_graphs=True)
+            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds, model), use_graphs=True)
             cells += int(rs.lengths().sum()) * V
             log(f"[bench] batched step {k}: {len(rs)} reads, {eng.last_stats['rows']} ticks, "
                 f"{time.perf_counter() - ts:.1f}s")
@@ -238,7 +244,6 @@ def main() -> int:
         log(f"[bench] cpu baseline: {cpu['value']} GCUPS on {cpu['cores']} threads, exact={cpu['exact']}")
 
     value = batch_gcups if batch_gcups is not None else plain_gcups
-    model = None
     ms = result.get("batch", result.get("plain"))["ms_per_step"]
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
@@ -247,7 +252,8 @@ def main() -> int:
         "config": {"workload": f"{wl.name}: 20 Mbp seeded DAG (SNP 1/25, indels 1-50 at 1/200), V={V} chars, "
                                f"n_s={info['n_s']}; reads log-normal median 3 kbp (1-30 kbp), 2% error, 1% N, "
                                f"mismatchN, score-only",
-                   "batch_reads": args.batch_reads, "plain_reads": args.plain_reads, "slots": args.slots},
+                   "batch_reads": args.batch_reads, "plain_reads": args.plain_reads, "slots": args.slots,
+                   "scheduler": scheduler},
         "plain_gcups": None if plain_gcups is None else round(plain_gcups, 3),
         "batched_gcups": None if batch_gcups is None else round(batch_gcups, 3),
         "speedup_batched_vs_plain": (round(batch_gcups / plain_gcups, 2) if batch_gcups and plain_gcups else None),

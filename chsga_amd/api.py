@@ -187,12 +187,15 @@ OP_DTYPE = np.dtype([("row", np.int64), ("vertex", np.uint32), ("op", np.uint8)]
 class Engine:
     """A chsga_ctx on one GPU."""
 
-    def __init__(self, ds: Dataset, alphabet: int = N.ALPHABET_MISMATCH_N, device: int = 0):
+    def __init__(self, ds: Dataset, alphabet: int = N.ALPHABET_MISMATCH_N, device: int = 0, desc=None):
+        """desc: a GraphDesc of ds to use instead of ds.desc(alphabet), e.g.
+        with other edit costs (the context copies what it needs)."""
         self.ds = ds
-        self.alphabet = alphabet
+        self.alphabet = alphabet if desc is None else int(desc.alphabet)
         self._lib = N.engine_lib()
         h = C.c_void_p()
-        N._check_engine(self._lib.chsga_ctx_create(C.byref(ds.desc(alphabet)), device, C.byref(h)))
+        d = ds.desc(alphabet) if desc is None else desc
+        N._check_engine(self._lib.chsga_ctx_create(C.byref(d), device, C.byref(h)))
         self._h = h
         self.last_stats: dict = {}
 

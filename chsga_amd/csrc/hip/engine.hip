@@ -60,11 +60,9 @@ using namespace chsga_dev;
 // Pull lists of the top set (sweep positions K0 .. n_s-1): up-edges from
 // below the set are external (sources are final when the top kernel starts),
 // the rest are internal and addressed by local index.
-// With by_pos, external pulls name their source by sweep position (the
-// batched path's F/Lst layout), else by original string id.
 static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<uint32_t>& up_off,
                       const std::vector<int2>& up_in, const std::vector<uint32_t>& dn_off,
-                      const std::vector<int2>& dn_in, const std::vector<uint32_t>& pos_of_og, bool by_pos) {
+                      const std::vector<int2>& dn_in, const std::vector<uint32_t>& pos_of_og) {
   T.j_top = j_top;
   T.K0 = j_top < c->J ? c->h_lvl_off[j_top] : uint32_t(c->n_s);
   T.NT = uint32_t(c->n_s) - T.K0;
@@ -80,10 +78,6 @@ static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<
       const auto lr = local(up_in[e]);
       if (lr.first)
         upi.push_back(lr.second);
-      else if (by_pos)
-        ext.push_back(make_int2(int32_t(pos_of_og[uint32_t(up_in[e].x) & 0x7fffffffu] |
-                                        (uint32_t(up_in[e].x) & 0x80000000u)),
-                                up_in[e].y));
       else
         ext.push_back(up_in[e]);
     }
@@ -104,13 +98,6 @@ static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<
   upload(T.dni, dni, c->stream);
   T.n_upi = uint32_t(upi.size());
   T.n_dni = uint32_t(dni.size());
-}
-
-// LDS bytes of k_bsweep_top_lds for a top set: F and Lst values of every slot,
-// then both internal pull lists with their offsets
-static size_t batch_top_lds_bytes(const TopSet& T) {
-  return size_t(2) * T.NT * 64 * sizeof(int32_t) + size_t(2) * (T.NT + 1) * sizeof(uint32_t) +
-         size_t(T.n_upi + T.n_dni) * sizeof(int2);
 }
 
 static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
@@ -250,8 +237,8 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     }
     dn_off[k + 1] = uint32_t(dn_in.size());
   }
-  // first step of the LDS-resident top kernel: as many top levels as fit
-  // kTopPlain (plain) / kTopBatch (batched) string vertices
+  // first step of the plain path's LDS-resident top kernel: as many top
+  // levels as fit kTopPlain string vertices
   auto top_step = [&](int64_t max_nt) {
     int32_t jt = J;
     while (jt > 0 && n_s - int64_t(c->h_lvl_off[jt - 1]) <= max_nt) --jt;
@@ -259,7 +246,6 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   };
   c->j_top = top_step(kTopPlain);
   if (c->top_level_opt >= 0) c->j_top = std::max(c->j_top, std::min<int32_t>(int32_t(c->top_level_opt), J));
-  const int32_t j_top_b = top_step(kTopBatch);
   std::vector<uint32_t> pos_of_og(n_s);
   for (int64_t og = 0; og < n_s; ++og) pos_of_og[og] = kpos[g->rank[og]];
 
@@ -273,21 +259,32 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_in_src, c->h_in_src, s);
   upload(c->d_first_bits, first_bits, s);
   if (c->compact) {
-    // up to two in-neighbour chars per first char; .x = -2 when there are more
-    std::vector<int2> nb2(V, make_int2(-1, -1));
+    // batched compact kernel: for every string, the last char (and, for a
+    // string of more than one char, the first char) of the at most two
+    // strings whose last char feeds its first char
+    std::vector<int4> bnb(n_s + 32, make_int4(-1, -1, -1, -1));
     c->nb2_all = true;
-    for (int64_t st = 0; st < n_s; ++st) {
+    for (int64_t st = 0; st < n_s && c->nb2_all; ++st) {
       const uint32_t p = c->h_str_off[st];
       const uint32_t e0 = c->h_in_off[p], e1 = c->h_in_off[p + 1];
       if (e1 - e0 > 2) {
-        nb2[p] = make_int2(-2, -1);
         c->nb2_all = false;
-      } else {
-        if (e1 > e0) nb2[p].x = int32_t(c->h_in_src[e0]);
-        if (e1 > e0 + 1) nb2[p].y = int32_t(c->h_in_src[e0 + 1]);
+        break;
       }
+      int32_t ent[4] = {-1, -1, -1, -1};
+      for (uint32_t e = e0; e < e1; ++e) {
+        const uint32_t u = c->h_in_src[e];
+        const int64_t t = int64_t(std::upper_bound(c->h_str_off.begin(), c->h_str_off.end(), u) -
+                                  c->h_str_off.begin()) - 1;  // string holding u
+        if (u + 1 != c->h_str_off[t + 1]) {  // not a last char: not an expansion after all
+          c->nb2_all = false;
+          break;
+        }
+        ent[2 * (e - e0)] = int32_t(u);
+        ent[2 * (e - e0) + 1] = c->h_str_off[t] == u ? -1 : int32_t(c->h_str_off[t]);
+      }
+      bnb[st] = make_int4(ent[0], ent[1], ent[2], ent[3]);
     }
-    upload(c->d_nb2, nb2, s);
     // label classes for the batched compact kernel (see read_class)
     std::vector<uint8_t> lclass(V);
     bool acgtn = true;
@@ -302,7 +299,17 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
       }
     }
     c->batch_fast = c->nb2_all && acgtn;
-    if (c->batch_fast) upload(c->d_lclass, lclass, s);
+    if (c->batch_fast) {
+      upload(c->d_lclass, lclass, s);
+      upload(c->d_bnb, bnb, s);
+    }
+  }
+  if (!c->batch_fast) {
+    // general batched kernel: first char of every char's string
+    std::vector<uint32_t> cfirst(V);
+    for (int64_t st = 0; st < n_s; ++st)
+      for (uint32_t p = c->h_str_off[st]; p < c->h_str_off[st + 1]; ++p) cfirst[p] = c->h_str_off[st];
+    upload(c->d_cfirst, cfirst, s);
   }
   upload(c->d_tile_char, tile_char, s);
   upload(c->d_tile_str, tile_str, s);
@@ -323,38 +330,21 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   }
   upload(c->d_dn_in, dn_in, s);
   c->ensure_staged_tables(g);
-  build_top(c, c->top_plain, c->j_top, up_off, up_in, dn_off, dn_in, pos_of_og, false);
+  build_top(c, c->top_plain, c->j_top, up_off, up_in, dn_off, dn_in, pos_of_og);
   {
-    // the batched top set also keeps its pull lists in LDS: shrink it until
-    // one workgroup's LDS holds everything
-    int lds_max = 0;
-    hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device), "attribute");
-    int32_t jt = j_top_b;
-    for (;;) {
-      build_top(c, c->top_batch, jt, up_off, up_in, dn_off, dn_in, pos_of_og, true);
-      if (c->top_batch.NT == 0 || batch_top_lds_bytes(c->top_batch) <= size_t(lds_max)) break;
-      ++jt;
-    }
-    c->top_batch_lds = batch_top_lds_bytes(c->top_batch);
-  }
-  {
-    auto by_pos = [&](const std::vector<int2>& in) {
+    // batched sweep: first char of the vertex at each sweep position, and the
+    // pull lists with the last char of each source string (padded)
+    auto by_last = [&](const std::vector<int2>& in) {
       std::vector<int2> out(in.size() + 2, make_int2(0, 0));  // 2 readable entries past the end
       for (size_t e = 0; e < in.size(); ++e)
-        out[e] = make_int2(int32_t(pos_of_og[uint32_t(in[e].x) & 0x7fffffffu] | (uint32_t(in[e].x) & 0x80000000u)),
-                           in[e].y);
+        out[e] = make_int2(int32_t(c->h_str_off[(uint32_t(in[e].x) & 0x7fffffffu) + 1] - 1), in[e].y);
       return out;
     };
-    upload(c->d_bup_in, by_pos(up_in), s);
-    upload(c->d_bdn_in, by_pos(dn_in), s);
-    std::vector<uint2> span(n_s + kSweepPad, make_uint2(0, 0));
-    for (int64_t k = 0; k < n_s; ++k) {
-      const uint32_t og = sw_vert[k];
-      const bool up_pulls = up_off[k + 1] > up_off[k];
-      span[k] = make_uint2(c->h_str_off[og] | (up_pulls ? 0x80000000u : 0u), c->h_str_off[og + 1]);
-    }
-    upload(c->d_bspan, span, s);
-    upload(c->d_bpos, pos_of_og, s);
+    upload(c->d_bup_in, by_last(up_in), s);
+    upload(c->d_bdn_in, by_last(dn_in), s);
+    std::vector<uint32_t> vfirst(n_s + kSweepPad, 0u);
+    for (int64_t k = 0; k < n_s; ++k) vfirst[k] = c->h_str_off[sw_vert[k]];
+    upload(c->d_vfirst, vfirst, s);
   }
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);

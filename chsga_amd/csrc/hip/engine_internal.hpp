@@ -42,7 +42,6 @@ constexpr int64_t kTopPlain = 4096;  // string vertices: 2 x 4 B each in LDS
 constexpr int64_t kSweepPad = 8;
 // chars the batched row buffers hold past V (>= one row-pass chunk)
 constexpr int64_t kRowPad = 32;
-constexpr int64_t kTopBatch = 0;     // string vertices: 2 x 64 slots x 4 B each in LDS
 struct TopSet {
   int32_t j_top = 0;
   uint32_t K0 = 0, NT = 0;
@@ -76,19 +75,16 @@ struct chsga_ctx {
   chsga_engine::DevMem d_sin_off, d_sin_src;
   // device graph
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
-  chsga_engine::DevMem d_nb2;  // compact mode: <= 2 in-neighbours of each first char
   bool nb2_all = false;  // compact, and no first char has more than 2 in-neighbours
   bool batch_fast = false;  // nb2_all and every label in ACGTN: the batched compact row kernel
   chsga_engine::DevMem d_lclass;  // match class of each char's label (batched compact kernel)
+  chsga_engine::DevMem d_bnb;     // per string: {last, first} chars of <= 2 in-neighbour strings
+  chsga_engine::DevMem d_cfirst;  // first char of each char's string (general batched kernel)
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
-  // batched path: F/Lst rows are stored by sweep position k, so a CH level is
-  // one contiguous range.  bpos[s] = k of string s; bup_in/bdn_in = the pull
-  // lists with positions in place of string ids; bspan[k] = {first char of
-  // the string | 1 << 31 when up-pulls may lower F below the row's first
-  // value, end char}
-  chsga_engine::DevMem d_bpos, d_bup_in, d_bdn_in, d_bspan;
-  TopSet top_plain, top_batch;
-  size_t top_batch_lds = 0;  // dynamic LDS bytes of the batched top kernel
+  // batched sweep: first char of the vertex at each sweep position; pull
+  // lists with the last char of each source string
+  chsga_engine::DevMem d_vfirst, d_bup_in, d_bdn_in;
+  TopSet top_plain;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
   // options and counters

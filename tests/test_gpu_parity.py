@@ -79,6 +79,26 @@ def test_slot_groups_match_oracle(slots):
     eng.close()
 
 
+@pytest.mark.parametrize("costs", [(0, 1, 1, 0), (0, 2, 3, 0), (1, 3, 2, 1)])
+def test_other_edit_costs(costs):
+    """Non-unit costs, including ins = 0, under which an insertion closure of
+    row 0 (which the reference does not run) would change the scores."""
+    from chsga_amd import native as N
+    ds, rs = small_workload(seed=29, n_reads=90, length=40, error=0.1, n_rate=0.02)
+    rs = ReadSet.from_list([rs[i][: 1 + (i * 7) % 40] for i in range(len(rs))])
+    d = N.GraphDesc.from_buffer_copy(ds.desc())
+    d.match_score, d.subst_score, d.del_score, d.ins_score = costs
+    ref = O.consga_scores(d, rs)
+    eng = Engine(ds, desc=d)
+    _say("costs", costs)
+    assert np.array_equal(eng.align_combo(rs), ref)
+    assert np.array_equal(eng.score_batch(rs), ref)
+    assert np.array_equal(eng.score_batch(rs, slots=96, use_graphs=True), ref)
+    for i in (0, 5, 17):
+        assert np.array_equal(eng.last_row(rs[i]), O.consga_last_row(d, rs[i]))
+    eng.close()
+
+
 def test_first_chars_with_three_in_neighbours(tmp_path):
     """A string with three predecessors takes the general row kernel of the
     batched path; its scores equal the oracle's."""
