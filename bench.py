@@ -49,16 +49,15 @@ def row_kernel_bytes(info: dict, score_bytes: int = 4) -> int:
 
 
 def batch_kernel_bytes(info: dict, score_bytes: int, slots: int = 64) -> int:
-    """Algorithmic bytes of one k_brow_tile launch (one tick: one DP row of
-    each of `slots` reads).  Per char: its slots' H read and D write; labels
-    and the first-char bit mask once for all slots; per string: the F and Lst
-    writes of every slot, and for its first char the in-CSR entry (8 B), the
-    in-neighbour ids (4 B each) and the in-neighbours' slot rows.  Chars that
-    are not first in their string take their in-neighbour p-1 from the row
-    just read (the char graph is the expansion of the string graph)."""
-    V, n_s, m_str = info["V"], info["n_s"], info["m_str"]
-    B, T = slots, score_bytes
-    return 2 * B * T * V + V + V // 8 + 2 * B * T * n_s + 8 * n_s + 4 * m_str + B * T * m_str
+    """Algorithmic (compulsory) bytes of one k_brow_tile launch (one tick: one
+    DP row of each of `slots` reads): every byte it must move once — the
+    slots' input row read and output row written (2 x slots x score bytes per
+    char), one label class byte and one first-char bit per char, and the
+    16-byte in-neighbour entry of every string.  In-neighbour rows gathered
+    for first chars are lines of the same input row (mostly L2 hits), so
+    they add no compulsory bytes; the measured traffic shows what they cost."""
+    V, n_s = info["V"], info["n_s"]
+    return 2 * slots * score_bytes * V + V + V // 8 + 16 * n_s
 
 
 def measured_traffic(bytes_per_launch: int):
