@@ -192,6 +192,13 @@ def main() -> int:
             rs = type(rs).from_list([rs[i][:64] for i in range(min(len(rs), 2 * args.slots))])
             eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds, model))
             log(f"[bench] batched warm-up {w} done")
+        # batched schedule from the autotuner (setup, outside the timed region)
+        tried = eng.autotune(step_reads(0, args.batch_reads), ticks=24)
+        best = min(tried, key=lambda t: t["us_per_tick"])
+        log(f"[bench] autotuned schedule {eng.schedule()} ({best['us_per_tick']:.0f} us/tick; "
+            f"{len(tried)} candidates, {tried[0]['us_per_tick']:.0f} us/tick for the default)")
+        result["schedule"] = eng.schedule()
+        result["autotune"] = [{k: (round(v, 1) if k == "us_per_tick" else v) for k, v in t.items()} for t in tried]
         cells = 0
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -252,7 +259,8 @@ def main() -> int:
                                f"n_s={info['n_s']}; reads log-normal median 3 kbp (1-30 kbp), 2% error, 1% N, "
                                f"mismatchN, score-only",
                    "batch_reads": args.batch_reads, "plain_reads": args.plain_reads, "slots": args.slots,
-                   "scheduler": scheduler},
+                   "scheduler": scheduler, "schedule": result.get("schedule")},
+        "autotune": result.get("autotune"),
         "plain_gcups": None if plain_gcups is None else round(plain_gcups, 3),
         "batched_gcups": None if batch_gcups is None else round(batch_gcups, 3),
         "speedup_batched_vs_plain": (round(batch_gcups / plain_gcups, 2) if batch_gcups and plain_gcups else None),

@@ -202,6 +202,27 @@ class Engine:
     def set_option(self, opt: int, value: int) -> None:
         N._check_engine(self._lib.chsga_ctx_set_option(self._h, opt, value))
 
+    # --- batched schedule and its autotuner ----------------------------------
+    def schedule(self) -> dict:
+        sc = N.BatchSchedule()
+        N._check_engine(self._lib.chsga_get_batch_schedule(self._h, C.byref(sc)))
+        return sc.as_dict()
+
+    def set_schedule(self, tile_chars: int, big_level: int, big_vpw: int) -> None:
+        sc = N.BatchSchedule(tile_chars, big_level, big_vpw)
+        N._check_engine(self._lib.chsga_set_batch_schedule(self._h, C.byref(sc)))
+
+    def autotune(self, reads: "ReadSet", ticks: int = 16) -> list:
+        """Times candidate batched schedules on these reads and keeps the
+        fastest (chsga_autotune_batch); returns every candidate tried as
+        dicts with its microseconds per tick."""
+        buf, arr, lens = reads.ptrs()
+        res = (N.TuneResult * 64)()
+        n = C.c_int32()
+        N._check_engine(self._lib.chsga_autotune_batch(self._h, arr, lens.ctypes.data_as(N.i64p), len(reads), ticks,
+                                                       res, 64, C.byref(n)))
+        return [dict(res[i].sched.as_dict(), us_per_tick=res[i].us_per_tick) for i in range(min(n.value, 64))]
+
     def align_combo(self, reads: ReadSet, per_read_times: bool = False):
         n = len(reads)
         buf, arr, lens = reads.ptrs()

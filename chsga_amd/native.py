@@ -54,6 +54,17 @@ class BatchOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("use_graphs", C.c_int32), ("order", i64p)]
 
 
+class BatchSchedule(C.Structure):
+    _fields_ = [("tile_chars", C.c_int32), ("big_level", C.c_int32), ("big_vpw", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("sched", BatchSchedule), ("us_per_tick", C.c_double)]
+
+
 class AlignOp(C.Structure):
     _fields_ = [("row", C.c_int64), ("vertex", C.c_uint32), ("op", C.c_uint8)]
 
@@ -102,6 +113,10 @@ ENGINE_SYMBOLS = {
     "chsga_submit_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, C.POINTER(BatchOpts),
                                      i32p]),
     "chsga_wait": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "chsga_get_batch_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule)]),
+    "chsga_set_batch_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule)]),
+    "chsga_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, C.c_int32,
+                                       C.POINTER(TuneResult), C.c_int32, C.POINTER(C.c_int32)]),
     "chsga_traceback_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int64, C.POINTER(C.POINTER(Path_)),
                                        C.POINTER(Stats)]),
     "chsga_path_free": (None, [C.POINTER(Path_)]),

@@ -159,6 +159,31 @@ typedef struct chsga_batch_opts {
 int chsga_score_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,
                       const chsga_batch_opts* opts, int32_t* scores_out, chsga_stats* stats);
 
+/* Schedule of the batched path: row-pass tile size and sweep launch shape.
+ * No counterpart in the reference (its row loop and sweep run on CPU
+ * threads); the values only change speed, never results. */
+typedef struct chsga_batch_schedule {
+  int32_t tile_chars; /* chars per row-pass tile, whole strings (32..4096); default 128 */
+  int32_t big_level;  /* CH levels with at least this many string vertices ... */
+  int32_t big_vpw;    /* ... take this many vertices per wave (1, 2, 4 or 8); smaller levels take 1 */
+} chsga_batch_schedule;
+int chsga_get_batch_schedule(chsga_ctx* ctx, chsga_batch_schedule* out);
+int chsga_set_batch_schedule(chsga_ctx* ctx, const chsga_batch_schedule* sched);
+
+/* Autotuner of the batched schedule: times `ticks` ticks (after a few
+ * untimed ones) of candidate schedules on 64 slots filled from the given
+ * reads (read chars are reused cyclically when a read is shorter than the
+ * run) and keeps the fastest, by coordinate descent over tile_chars, then
+ * big_vpw, then big_level.  results[max_results] receives every candidate
+ * tried with its microseconds per tick; *n_results their number.  Scores are
+ * not computed.  No counterpart in the reference. */
+typedef struct chsga_tune_result {
+  chsga_batch_schedule sched;
+  double us_per_tick;
+} chsga_tune_result;
+int chsga_autotune_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n, int32_t ticks,
+                         chsga_tune_result* results, int32_t max_results, int32_t* n_results);
+
 /* Asynchronous form of chsga_score_batch: the work runs on a host worker
  * thread; chsga_wait joins it.  The arrays must stay valid until chsga_wait. */
 int chsga_submit_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,

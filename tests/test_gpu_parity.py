@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from chsga_amd import ALPHABET_MATCH_N, ALPHABET_MISMATCH_N, Dataset, Engine, ReadSet, WORKLOADS, small_workload
+from chsga_amd import native as N
 from chsga_amd.scheduler import plan_order
 from oracle import oracle as O
 from tests import refmodels as R
@@ -83,7 +84,6 @@ def test_slot_groups_match_oracle(slots):
 def test_other_edit_costs(costs):
     """Non-unit costs, including ins = 0, under which an insertion closure of
     row 0 (which the reference does not run) would change the scores."""
-    from chsga_amd import native as N
     ds, rs = small_workload(seed=29, n_reads=90, length=40, error=0.1, n_rate=0.02)
     rs = ReadSet.from_list([rs[i][: 1 + (i * 7) % 40] for i in range(len(rs))])
     d = N.GraphDesc.from_buffer_copy(ds.desc())
@@ -96,6 +96,34 @@ def test_other_edit_costs(costs):
     assert np.array_equal(eng.score_batch(rs, slots=96, use_graphs=True), ref)
     for i in (0, 5, 17):
         assert np.array_equal(eng.last_row(rs[i]), O.consga_last_row(d, rs[i]))
+    eng.close()
+
+
+@pytest.mark.parametrize("sched", [(32, 1, 1), (64, 1, 2), (512, 1, 8), (128, 10 ** 9, 4), (4096, 16, 4)])
+def test_batch_schedules_are_exact(sched):
+    """Every tile size and sweep launch shape gives the oracle's scores."""
+    ds, rs = small_workload(seed=37, n_reads=70, length=60, error=0.06, n_rate=0.02)
+    ref = O.consga_scores(ds.desc(), rs)
+    eng = Engine(ds)
+    eng.set_schedule(*sched)
+    assert eng.schedule() == dict(tile_chars=sched[0], big_level=sched[1], big_vpw=sched[2])
+    assert np.array_equal(eng.score_batch(rs), ref)
+    assert np.array_equal(eng.score_batch(rs, use_graphs=True), ref)
+    eng.close()
+
+
+def test_autotune_keeps_the_fastest_schedule():
+    ds, rs = small_workload(seed=38, n_reads=40, length=80, error=0.04)
+    eng = Engine(ds)
+    tried = eng.autotune(rs, ticks=8)
+    assert len(tried) >= 10 and all(t["us_per_tick"] > 0 for t in tried)
+    best = min(tried, key=lambda t: t["us_per_tick"])
+    assert eng.schedule() == {k: best[k] for k in ("tile_chars", "big_level", "big_vpw")}
+    assert np.array_equal(eng.score_batch(rs), O.consga_scores(ds.desc(), rs))
+    with pytest.raises(N.NativeError):
+        eng.set_schedule(16, 1, 4)
+    with pytest.raises(N.NativeError):
+        eng.set_schedule(128, 1, 3)
     eng.close()
 
 
