@@ -273,6 +273,36 @@ class Engine:
         finally:
             self._lib.chsga_path_free(p)
 
+    def mem_info(self) -> tuple:
+        """(free, total) device bytes of the context's GPU."""
+        fr, to = C.c_int64(), C.c_int64()
+        N._check_engine(self._lib.chsga_mem_info(self._h, C.byref(fr), C.byref(to)))
+        return int(fr.value), int(to.value)
+
+    def traceback_stream(self, reads: "ReadSet", model=None, budget_bytes: int | None = None,
+                         candidates=(2000, 1000, 500, 250, 125, 64, 32, 16)) -> list:
+        """Checkpointed traceback of every read, each with its own interval k:
+        the largest candidate whose peak device bytes fit the budget (default:
+        half the free device memory).  The learned cost model proposes k from
+        its predicted bytes; the exact byte formula (learned.traceback_peak_bytes)
+        checks the proposal and lowers k while it does not fit, and alone picks
+        k when no model is given.  Returns (TracebackResult, k) per read."""
+        from . import learned as L
+        if budget_bytes is None:
+            budget_bytes = self.mem_info()[0] // 2
+        V = self.ds.info()["V"]
+        out = []
+        for i in range(len(reads)):
+            r = reads[i]
+            k = model.choose_k(r, self.ds, budget_bytes, candidates) if model is not None else candidates[0]
+            ks = [c for c in candidates if c <= k] or [candidates[-1]]
+            for c in ks:  # the formula is the judge: first candidate at or below k that fits
+                k = c
+                if L.traceback_peak_bytes(len(r), V, c) <= budget_bytes:
+                    break
+            out.append((self.align_read_with_traceback(r, k=k), k))
+        return out
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             self._lib.chsga_ctx_destroy(self._h)

@@ -537,6 +537,20 @@ int chsga_device_count(int* count) {
   })
 }
 
+int chsga_mem_info(chsga_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+  if (!ctx || !free_bytes || !total_bytes) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  ENGINE_GUARD({
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    size_t fr = 0, to = 0;
+    hip_check(hipMemGetInfo(&fr, &to), "hipMemGetInfo");
+    *free_bytes = int64_t(fr);
+    *total_bytes = int64_t(to);
+  })
+}
+
 int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out) {
   if (!out) {
     set_error("null output pointer");

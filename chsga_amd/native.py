@@ -103,6 +103,7 @@ ENGINE_SYMBOLS = {
     "chsga_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "chsga_ctx_create": (C.c_int, [C.POINTER(GraphDesc), C.c_int, C.POINTER(C.c_void_p)]),
     "chsga_ctx_destroy": (None, [C.c_void_p]),
+    "chsga_mem_info": (C.c_int, [C.c_void_p, i64p, i64p]),
     "chsga_ctx_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
     "chsga_score_reads": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, i32p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),

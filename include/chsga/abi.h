@@ -132,6 +132,11 @@ typedef struct chsga_stats {
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);
 
+/* Free and total device memory of the context's GPU, in bytes: the budget
+ * for choosing a traceback's checkpoint interval.  No counterpart in the
+ * reference (its checkpoints go to disk). */
+int chsga_mem_info(chsga_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+
 int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out);
 void chsga_ctx_destroy(chsga_ctx* ctx);
 int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value);

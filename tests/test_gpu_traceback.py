@@ -81,3 +81,27 @@ def test_c1_traceback_k500(c1):
         assert O.validate_path(d, rs[i], tb.ops, tb.score)[0] == 0
         assert tb.cigar()[-1] in "MDI"
     eng.close()
+
+
+def test_traceback_stream_fits_the_budget_with_learned_k():
+    """The learned model proposes each read's checkpoint interval from its
+    predicted peak bytes; the exact formula checks it; every traceback stays
+    within the budget (measured), and its path is valid and optimal."""
+    from chsga_amd import learned as L
+    ds, rs = small_workload(seed=41, n_reads=5, length=300, error=0.05)
+    d = ds.desc()
+    V = ds.info()["V"]
+    eng = Engine(ds)
+    scores = eng.align_combo(rs)
+    budget = L.traceback_peak_bytes(300, V, 64)  # intervals above 64 do not fit a 300-char read
+    model = L.load_default()
+    for use_model in (False, True):
+        out = eng.traceback_stream(rs, model=model if use_model else None, budget_bytes=budget)
+        for i, (tb, k) in enumerate(out):
+            assert k <= 64 and L.traceback_peak_bytes(len(rs[i]), V, k) <= budget
+            assert tb.score == scores[i]
+            assert O.validate_path(d, rs[i], tb.ops, tb.score)[0] == 0
+        assert eng.last_stats["peak_device_bytes"] <= budget
+    if model is not None:
+        model.close()
+    eng.close()
