@@ -273,6 +273,33 @@ class Engine:
         finally:
             self._lib.chsga_path_free(p)
 
+    def traceback_batch(self, reads: "ReadSet", k: int = 500, slots: int = 64, order=None, use_graphs: bool = False,
+                        budget_bytes: int = 0) -> list:
+        """Checkpointed traceback of every read with the batched forward pass
+        (chsga_traceback_batch); one TracebackResult per read."""
+        n = len(reads)
+        buf, arr, lens = reads.ptrs()
+        ordv = None if order is None else np.ascontiguousarray(order, dtype=np.int64)
+        opts = N.BatchOpts(slots, 1 if use_graphs else 0, ordv.ctypes.data_as(N.i64p) if ordv is not None else None)
+        paths = (C.POINTER(N.Path_) * n)()
+        st = N.Stats()
+        N._check_engine(self._lib.chsga_traceback_batch(self._h, arr, lens.ctypes.data_as(N.i64p), n, k,
+                                                        C.byref(opts), budget_bytes, paths, C.byref(st)))
+        self.last_stats = st.as_dict()
+        out = []
+        try:
+            for i in range(n):
+                p = paths[i]
+                m = p.contents.n_ops
+                raw = C.string_at(p.contents.ops, m * C.sizeof(N.AlignOp))
+                out.append(TracebackResult(int(p.contents.score), int(p.contents.end_vertex),
+                                           np.frombuffer(raw, dtype=OP_DTYPE).copy()))
+        finally:
+            for i in range(n):
+                if paths[i]:
+                    self._lib.chsga_path_free(paths[i])
+        return out
+
     def mem_info(self) -> tuple:
         """(free, total) device bytes of the context's GPU."""
         fr, to = C.c_int64(), C.c_int64()

@@ -121,6 +121,9 @@ ENGINE_SYMBOLS = {
     "chsga_traceback_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int64, C.POINTER(C.POINTER(Path_)),
                                        C.POINTER(Stats)]),
     "chsga_path_free": (None, [C.POINTER(Path_)]),
+    "chsga_traceback_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, C.c_int64,
+                                        C.POINTER(BatchOpts), C.c_int64, C.POINTER(C.POINTER(Path_)),
+                                        C.POINTER(Stats)]),
     "chsga_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_int,
                                    C.POINTER(C.c_void_p)]),
     "chsga_mlp_destroy": (None, [C.c_void_p]),

@@ -217,6 +217,18 @@ int chsga_traceback_read(chsga_ctx* ctx, const char* read, int64_t len, int64_t 
                          chsga_stats* stats);
 void chsga_path_free(chsga_path* p);
 
+/* Batched checkpointed traceback: the batched forward pass (slots, device
+ * work queue, graphs as in chsga_score_batch) keeps rows 0, k, 2k, ... of
+ * every read and the argmin of its last row in HBM; each read's path is then
+ * recovered by segment recompute and walk on the GPU, as in
+ * chsga_traceback_read.  Reads enter in opts->order, in waves whose
+ * checkpoint rows fit budget_bytes (0: half the free device memory after the
+ * batch buffers); a read whose rows alone exceed it is an error (use a larger
+ * k).  paths[n] receives one path per read (free each with chsga_path_free).
+ * Replaces align_combo_with_traceback (src/traceback.hpp:1040-1053). */
+int chsga_traceback_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n, int64_t k,
+                          const chsga_batch_opts* opts, int64_t budget_bytes, chsga_path** paths, chsga_stats* stats);
+
 /*
  * Learned cost model (no counterpart in the reference, which schedules
  * nothing): an FP32 multilayer perceptron in -> h1 -> h2 -> out with ReLU

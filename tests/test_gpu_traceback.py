@@ -105,3 +105,53 @@ def test_traceback_stream_fits_the_budget_with_learned_k():
     if model is not None:
         model.close()
     eng.close()
+
+
+@pytest.mark.parametrize("k", [1, 7, 500])
+def test_batched_traceback_equals_oracle_walk(k):
+    """The batched forward pass keeps every k-th row and the last row's
+    argmin; the recovered paths equal the oracle's walk over the full score
+    matrix, read by read."""
+    ds, rs = small_workload(seed=43, n_reads=70, length=60, error=0.08, n_rate=0.02)
+    rs = ReadSet.from_list([rs[i][: 1 + (i * 11) % 60] for i in range(len(rs))])
+    d = ds.desc()
+    eng = Engine(ds)
+    out = eng.traceback_batch(rs, k=k)
+    scores = eng.align_combo(rs)
+    for i, tb in enumerate(out):
+        assert tb.score == scores[i]
+        osc, ops = O.traceback(d, rs[i])
+        assert osc == tb.score and np.array_equal(ops, tb.ops), (k, i)
+    eng.close()
+
+
+def test_batched_traceback_waves_groups_and_matchN():
+    """A budget of a few reads' checkpoint rows forces several waves; slot
+    groups, graph replay and matchN give the same paths as the per-read
+    traceback."""
+    from chsga_amd import learned as L
+    ds, rs = small_workload(seed=44, n_reads=40, length=90, error=0.06, n_rate=0.05)
+    d = ds.desc(ALPHABET_MATCH_N)
+    eng = Engine(ds, ALPHABET_MATCH_N)
+    V = ds.info()["V"]
+    budget = 3 * L.traceback_peak_bytes(90, V, 25)
+    out = eng.traceback_batch(rs, k=25, slots=96, use_graphs=True, budget_bytes=budget)
+    for i, tb in enumerate(out):
+        ref = eng.align_read_with_traceback(rs[i], k=25)
+        assert tb.score == ref.score and np.array_equal(tb.ops, ref.ops)
+        assert O.validate_path(d, rs[i], tb.ops, tb.score)[0] == 0
+    with pytest.raises(N.NativeError):
+        eng.traceback_batch(rs, k=25, budget_bytes=1000)
+    eng.close()
+
+
+def test_c1_batched_traceback(c1):
+    w, ds, rs = c1
+    d = ds.desc(w.alphabet)
+    eng = Engine(ds, w.alphabet)
+    out = eng.traceback_batch(rs, k=w.traceback_k, use_graphs=True)
+    scores = eng.score_batch(rs)
+    for i, tb in enumerate(out):
+        assert tb.score == scores[i]
+        assert O.validate_path(d, rs[i], tb.ops, tb.score)[0] == 0
+    eng.close()
