@@ -120,6 +120,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run a torch copy of N MiB (PMC scaling)")
+    ap.add_argument("--traceback", type=int, default=0,
+                    help="also time checkpointed traceback (k=500) of this many reads: batched and per read")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -244,6 +246,27 @@ def main() -> int:
     log(f"[bench] k_brow_tile: {b_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
         f"k_row_tile: {p_ms * 1e3:.1f} us/launch")
 
+    # ---- optional: checkpointed traceback (paths), batched and per read ----
+    if args.traceback > 0:
+        rs = step_reads(0, args.traceback)
+        cells = int(rs.lengths().sum()) * V
+        t0 = time.perf_counter()
+        paths = eng.traceback_batch(rs, k=500, order=plan_order(rs, ds, model), use_graphs=True)
+        dt = time.perf_counter() - t0
+        sc = eng.score_batch(rs)
+        ok = all(p.score == int(sc[i]) for i, p in enumerate(paths))
+        n1 = min(2, len(rs))
+        t1 = time.perf_counter()
+        for i in range(n1):
+            eng.align_read_with_traceback(rs[i], k=500)
+        dt1 = time.perf_counter() - t1
+        cells1 = int(rs.lengths()[:n1].sum()) * V
+        result["traceback"] = {"reads": len(rs), "k": 500, "batched_gcups": round(cells / dt / 1e9, 3),
+                               "batched_s": round(dt, 2), "per_read_gcups": round(cells1 / dt1 / 1e9, 3),
+                               "scores_match": ok}
+        log(f"[bench] traceback: batched {cells / dt / 1e9:.2f} GCUPS ({len(rs)} reads, {dt:.1f}s), "
+            f"per read {cells1 / dt1 / 1e9:.2f} GCUPS, scores match: {ok}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds, wl, reads, args.cpu_budget, eng)
@@ -266,6 +289,8 @@ def main() -> int:
         "speedup_batched_vs_plain": (round(batch_gcups / plain_gcups, 2) if batch_gcups and plain_gcups else None),
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if "traceback" in result:
+        line["traceback"] = result["traceback"]
     print(json.dumps(line), flush=True)
     return 0
 

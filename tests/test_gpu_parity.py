@@ -251,6 +251,10 @@ def test_drop_in_drivers(tmp_path):
                          text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr
     assert "[RESULT] ALL CHECKS PASSED" in out.stdout and "(checkpoint every k=100 rows" in out.stdout
+    out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-k", "100", "-B"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    assert "[RESULT] ALL CHECKS PASSED" in out.stdout
     out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-b"], capture_output=True, text=True,
                          timeout=300)
     assert out.returncode == 0 and " Substitutions: NA sec" in out.stdout
