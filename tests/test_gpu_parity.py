@@ -218,6 +218,29 @@ def test_c3_full_size_properties():
     eng.close()
 
 
+def test_c5_large_graph_matchN():
+    """The largest workload's graph (116.7M chars, 28 CH levels, structural
+    bubbles, matchN): row offsets exceed 2^31 elements, so every index is
+    64-bit.  Batched and plain scores agree on short reads; every score lies in
+    [0, |r|]; error-free reads along graph paths score 0."""
+    w = WORKLOADS["c5"]
+    ds = w.dataset()
+    rs = w.readset(ds, n_reads=10)
+    reads = ReadSet.from_list([rs[i][:1500] for i in range(10)])
+    clean = ReadSet.generate(ds, seed=7, n_reads=2, length=500)
+    both = ReadSet.from_list([reads[i] for i in range(10)] + [clean[0], clean[1]])
+    eng = Engine(ds, w.alphabet)
+    _say("c5 batched", both.lengths().tolist())
+    b = eng.score_batch(both)
+    assert (b >= 0).all() and (b <= both.lengths()).all()
+    assert b[10] == 0 and b[11] == 0
+    short = [i for i in range(len(both)) if both.lengths()[i] <= 300][:3]
+    _say("c5 plain", short)
+    p = eng.align_combo(both.subset(short))
+    assert np.array_equal(p, b[short])
+    eng.close()
+
+
 def test_drop_in_drivers(tmp_path):
     """The ConSGA_* binaries on written input files: the score line equals the
     oracle's, the traceback driver's verification passes, and flags the
