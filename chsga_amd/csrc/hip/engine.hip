@@ -334,17 +334,54 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   {
     // batched sweep: first char of the vertex at each sweep position, and the
     // pull lists with the last char of each source string (padded)
+    // longest pull list per level; the sweep loads a vertex's pulls in
+    // chunks and may read up to (longest list + chunk) entries past the
+    // start of any list of its level, hence the padding
+    auto maxp = [&](const std::vector<uint32_t>& off, std::vector<uint32_t>& out) {
+      out.assign(size_t(c->J), 0u);
+      uint32_t all = 0;
+      for (int32_t j = 0; j < c->J; ++j)
+        for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) out[j] = std::max(out[j], off[k + 1] - off[k]);
+      for (uint32_t m : out) all = std::max(all, m);
+      return all;
+    };
+    const uint32_t pad = std::max(maxp(up_off, c->h_up_maxp), maxp(dn_off, c->h_dn_maxp)) + 2 * kSweepChunk;
     auto by_last = [&](const std::vector<int2>& in) {
-      std::vector<int2> out(in.size() + 2, make_int2(0, 0));  // 2 readable entries past the end
+      std::vector<int2> out(in.size() + pad, make_int2(0, 0));
       for (size_t e = 0; e < in.size(); ++e)
         out[e] = make_int2(int32_t(c->h_str_off[(uint32_t(in[e].x) & 0x7fffffffu) + 1] - 1), in[e].y);
       return out;
     };
-    upload(c->d_bup_in, by_last(up_in), s);
-    upload(c->d_bdn_in, by_last(dn_in), s);
+    const std::vector<int2> bup = by_last(up_in), bdn = by_last(dn_in);
+    upload(c->d_bup_in, bup, s);
+    upload(c->d_bdn_in, bdn, s);
     std::vector<uint32_t> vfirst(n_s + kSweepPad, 0u);
     for (int64_t k = 0; k < n_s; ++k) vfirst[k] = c->h_str_off[sw_vert[k]];
     upload(c->d_vfirst, vfirst, s);
+    // levels whose lists hold at most two pulls of weight < 2^16: one 16-byte
+    // record per vertex {first, src0, src1, w0 | w1 << 16}; a missing pull
+    // is the vertex's own first char with weight 0
+    auto pairs = [&](const std::vector<uint32_t>& off, const std::vector<int2>& pin,
+                     const std::vector<uint32_t>& lmax, std::vector<uint8_t>& ok) {
+      std::vector<uint4> rec(n_s + kSweepPad, make_uint4(0, 0, 0, 0));
+      ok.assign(size_t(c->J), 0);
+      for (int32_t j = 0; j < c->J; ++j) {
+        bool fits = lmax[j] <= 2;
+        for (uint32_t e = off[c->h_lvl_off[j]]; fits && e < off[c->h_lvl_off[j + 1]]; ++e)
+          fits = pin[e].y >= 0 && pin[e].y < 65536;
+        ok[j] = fits;
+        if (!fits) continue;
+        for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) {
+          const uint32_t n = off[k + 1] - off[k], f = vfirst[k];
+          const uint32_t s0 = n > 0 ? uint32_t(pin[off[k]].x) : f, s1 = n > 1 ? uint32_t(pin[off[k] + 1].x) : f;
+          const uint32_t w0 = n > 0 ? uint32_t(pin[off[k]].y) : 0u, w1 = n > 1 ? uint32_t(pin[off[k] + 1].y) : 0u;
+          rec[k] = make_uint4(f, s0, s1, w0 | (w1 << 16));
+        }
+      }
+      return rec;
+    };
+    upload(c->d_bup_rec, pairs(up_off, bup, c->h_up_maxp, c->h_up_pair), s);
+    upload(c->d_bdn_rec, pairs(dn_off, bdn, c->h_dn_maxp, c->h_dn_pair), s);
   }
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);

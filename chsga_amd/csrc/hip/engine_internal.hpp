@@ -40,6 +40,9 @@ struct chsga_batch_state;
 constexpr int64_t kTopPlain = 4096;  // string vertices: 2 x 4 B each in LDS
 // entries the batched level sweep may read past a table's end (>= its vertices per wave)
 constexpr int64_t kSweepPad = 8;
+// largest number of pull entries the batched level sweep loads per vertex at
+// once (one vertex per wave, on levels whose lists exceed two entries)
+constexpr int kSweepChunk = 16;
 // chars the batched row buffers hold past V (>= one row-pass chunk)
 constexpr int64_t kRowPad = 32;
 struct TopSet {
@@ -86,6 +89,10 @@ struct chsga_ctx {
   // batched sweep: first char of the vertex at each sweep position; pull
   // lists with the last char of each source string
   chsga_engine::DevMem d_vfirst, d_bup_in, d_bdn_in;
+  std::vector<uint32_t> h_up_maxp, h_dn_maxp;  // longest pull list of each level
+  // per vertex {first, src0, src1, w0 | w1 << 16} on levels flagged in h_*_pair
+  chsga_engine::DevMem d_bup_rec, d_bdn_rec;
+  std::vector<uint8_t> h_up_pair, h_dn_pair;
   TopSet top_plain;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
