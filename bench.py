@@ -253,6 +253,7 @@ def main() -> int:
         t0 = time.perf_counter()
         paths = eng.traceback_batch(rs, k=500, order=plan_order(rs, ds, model), use_graphs=True)
         dt = time.perf_counter() - t0
+        tst = dict(eng.last_stats)
         sc = eng.score_batch(rs)
         ok = all(p.score == int(sc[i]) for i, p in enumerate(paths))
         n1 = min(2, len(rs))
@@ -263,7 +264,9 @@ def main() -> int:
         cells1 = int(rs.lengths()[:n1].sum()) * V
         result["traceback"] = {"reads": len(rs), "k": 500, "batched_gcups": round(cells / dt / 1e9, 3),
                                "batched_s": round(dt, 2), "per_read_gcups": round(cells1 / dt1 / 1e9, 3),
-                               "scores_match": ok}
+                               "scores_match": ok, "ticks": tst.get("rows"),
+                               "walk_fallbacks": tst.get("walk_fallbacks"),
+                               "peak_device_gb": round(tst.get("peak_device_bytes", 0) / 1e9, 1)}
         log(f"[bench] traceback: batched {cells / dt / 1e9:.2f} GCUPS ({len(rs)} reads, {dt:.1f}s), "
             f"per read {cells1 / dt1 / 1e9:.2f} GCUPS, scores match: {ok}")
 

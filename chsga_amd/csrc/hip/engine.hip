@@ -638,6 +638,9 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
       case CHSGA_OPT_STAGED_WALK:
         ctx->force_staged = value != 0;
         break;
+      case CHSGA_OPT_BATCHED_BACKWARD:
+        ctx->batched_backward = value != 0;
+        break;
       default:
         throw std::invalid_argument("unknown option");
     }

@@ -98,6 +98,7 @@ struct chsga_ctx {
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
   // options and counters
   bool profile = false, use_graphs = false, force_staged = false;
+  bool batched_backward = true;  // batched traceback: lock-step segment recompute when it fits
   bool compact = false;  // char graph is the expansion of the string graph
   bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)
   int64_t launches = 0, rows = 0;

@@ -18,7 +18,7 @@ CHSGA_OK = 0
 ALPHABET_MATCH_N = 4
 ALPHABET_MISMATCH_N = 5
 OP_DELETE, OP_MATCHSUB, OP_INSERT = 0, 1, 2
-OPT_PROFILE, OPT_USE_GRAPHS, OPT_STAGED_WALK, OPT_NARROW = 1, 2, 5, 6
+OPT_PROFILE, OPT_USE_GRAPHS, OPT_STAGED_WALK, OPT_NARROW, OPT_BATCHED_BACKWARD = 1, 2, 5, 6, 7
 
 u64p = C.POINTER(C.c_uint64)
 u32p = C.POINTER(C.c_uint32)

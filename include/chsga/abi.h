@@ -128,6 +128,8 @@ typedef struct chsga_stats {
 #define CHSGA_OPT_USE_GRAPHS 2   /* 1: replay captured hipGraphs per rows     */
 #define CHSGA_OPT_STAGED_WALK 5  /* 1: walk every traceback row through its stages (validation) */
 #define CHSGA_OPT_NARROW 6       /* 1 (default): store scores as int16 when provably < 2^15; 0: int32 */
+#define CHSGA_OPT_BATCHED_BACKWARD 7 /* 1 (default): chsga_traceback_batch recomputes segments of 64 reads
+                                        at once when its row buffers fit; 0: read by read */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);

@@ -155,3 +155,52 @@ def test_c1_batched_traceback(c1):
         assert tb.score == scores[i]
         assert O.validate_path(d, rs[i], tb.ops, tb.score)[0] == 0
     eng.close()
+
+
+@pytest.mark.parametrize("k", [3, 13, 64])
+def test_batched_backward_equals_per_read_backward(k):
+    """Lock-step segment recompute (up to 64 reads a round, snapshots every
+    k2 = ceil(sqrt(k)) rows, a ring of k2 rows, all lanes walked at once)
+    gives the paths of recomputing each read's segments alone: more reads
+    than slots, uneven lengths (1..200 chars), int16 and int32 rows."""
+    ds, rs = small_workload(seed=45, n_reads=150, length=200, error=0.08, n_rate=0.02)
+    rs = ReadSet.from_list([rs[i][: 1 + (i * 37) % 200] for i in range(len(rs))])
+    eng = Engine(ds)
+    print("batched backward", k, flush=True)
+    a = eng.traceback_batch(rs, k=k)
+    eng.set_option(N.OPT_BATCHED_BACKWARD, 0)
+    print("per-read backward", k, flush=True)
+    b = eng.traceback_batch(rs, k=k)
+    eng.set_option(N.OPT_BATCHED_BACKWARD, 1)
+    eng.set_option(N.OPT_NARROW, 0)
+    print("batched backward, int32 rows", k, flush=True)
+    c = eng.traceback_batch(rs, k=k)
+    for i in range(len(rs)):
+        assert a[i].score == b[i].score == c[i].score
+        assert np.array_equal(a[i].ops, b[i].ops) and np.array_equal(a[i].ops, c[i].ops), (k, i)
+    eng.close()
+
+
+def test_batched_backward_staged_walk_gives_valid_paths(tmp_path):
+    """The batched backward with every row walked through its stages: lane
+    rows are made contiguous for the staged walk; paths are valid and of the
+    read's score (random string DAGs, first chars with many in-neighbours)."""
+    rng = np.random.default_rng(9)
+    for t in range(3):
+        strings, edges = R.random_string_dag(rng, 30, 5, p_edge=0.08)
+        p = R.write_dataset(tmp_path, strings, edges, f"bsw{t}")
+        ds = Dataset.load(p["char.adj"], p["str.adj"], None, p["labels"], p["label_offsets"])
+        d = ds.desc()
+        eng = Engine(ds)
+        reads = [R.rbytes(rng, int(rng.integers(1, 30))) for _ in range(5)]
+        rs = ReadSet.from_list(reads)
+        scores = eng.align_combo(rs)
+        eng.set_option(N.OPT_STAGED_WALK, 1)
+        print("batched staged walk", t, flush=True)
+        out = eng.traceback_batch(rs, k=4)
+        assert eng.last_stats["walk_fallbacks"] == sum(len(r) - 1 for r in reads)
+        for i, tb in enumerate(out):
+            assert tb.score == scores[i]
+            rc, bad = O.validate_path(d, reads[i], tb.ops, tb.score)
+            assert rc == 0, (t, i, rc, bad)
+        eng.close()
