@@ -3,12 +3,17 @@
 
 A step is one pass of the hot path over one batch of synthetic reads of the
 c3 workload (20 Mbp seeded chain-of-bubbles DAG, ~22M char vertices; reads
-log-normal median 3 kbp clipped to 1-30 kbp, 2% error, 1% N, mismatchN).
-  value      GCUPS of the batched path (slot-interleaved rows, device work
-             queue, reads packed in order of the learned cost model), whole
-             job, inputs resident in HBM;
-  plain      GCUPS of the plain path (one read per launch sequence) on the
-             same graph;
+log-normal median 3 kbp clipped to 1-30 kbp, 2% error, 1% N, mismatchN):
+the optimal score and an alignment path of every read, with checkpoint rows
+every k = 500 kept in HBM.
+  value      GCUPS of the batched path with traceback (slot-interleaved rows,
+             device work queue, reads packed in order of the learned cost
+             model; forward pass keeping every 500th row, then segment
+             recompute and walk of 64 reads at a time), whole job, inputs
+             resident in HBM; score_only_gcups is its forward pass alone
+             (--score-only times score-only batches instead);
+  plain      GCUPS of the plain path (one read per launch sequence,
+             score-only) on the same graph;
   roofline   the batched row kernel (k_brow_tile, one DP row of 64 reads per
              launch): algorithmic bytes per launch over its mean duration from
              HIP events on the engine's stream; traffic = HBM bytes per launch
@@ -101,7 +106,8 @@ def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
     gpu_score = gpu_engine.align_combo(sample)
     return {
         "value": round(rows * V / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
-        "sample": f"read 0 of the c3 read set truncated to its first {rows} chars ({rows} DP rows x {V} chars), "
+        "sample": f"score-only DP of read 0 of the c3 read set truncated to its first {rows} chars "
+                  f"({rows} DP rows x {V} chars; paths are not timed on the CPU), "
                   f"{dt:.1f} s on {threads} OpenMP threads",
         "score_cpu": int(cpu_score[0]), "score_gpu": int(gpu_score[0]), "exact": bool(cpu_score[0] == gpu_score[0]),
     }
@@ -113,7 +119,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--batch-reads", type=int, default=384, help="reads per batched step")
+    ap.add_argument("--batch-reads", type=int, default=256, help="reads per batched step")
+    ap.add_argument("--k", type=int, default=500, help="checkpoint interval of the traceback")
+    ap.add_argument("--score-only", action="store_true", help="batched steps compute scores only (no paths)")
     ap.add_argument("--plain-reads", type=int, default=2, help="reads per plain step")
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
@@ -121,7 +129,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run a torch copy of N MiB (PMC scaling)")
     ap.add_argument("--traceback", type=int, default=0,
-                    help="also time checkpointed traceback (k=500) of this many reads: batched and per read")
+                    help="also time the per-read checkpointed traceback of this many reads")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -185,14 +193,17 @@ def main() -> int:
         log(f"[bench] plain: {plain_gcups:.2f} GCUPS ({dt:.2f}s, {args.steps} steps)")
 
     # ---- batched path -------------------------------------------------------
-    # A step is one stream of batch_reads reads through the 64 slots; the
-    # warm-up is a short batch that allocates the slot buffers.
-    batch_gcups = None
+    # A step is one stream of batch_reads reads through the 64 slots: scores
+    # and paths (checkpointed traceback), or scores only with --score-only;
+    # the warm-up is a short batch that allocates the slot buffers.
+    batch_gcups = score_gcups = None
     if args.mode in ("both", "batch"):
         for w in range(args.warmup):
             rs = step_reads(w, args.batch_reads)
             rs = type(rs).from_list([rs[i][:64] for i in range(min(len(rs), 2 * args.slots))])
             eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds, model))
+            if not args.score_only:
+                eng.traceback_batch(rs, k=args.k, order=plan_order(rs, ds, model))
             log(f"[bench] batched warm-up {w} done")
         # batched schedule from the autotuner (setup, outside the timed region)
         tried = eng.autotune(step_reads(0, args.batch_reads), ticks=24)
@@ -201,20 +212,38 @@ def main() -> int:
             f"{len(tried)} candidates, {tried[0]['us_per_tick']:.0f} us/tick for the default)")
         result["schedule"] = eng.schedule()
         result["autotune"] = [{k: (round(v, 1) if k == "us_per_tick" else v) for k, v in t.items()} for t in tried]
-        cells = 0
+        cells = ticks = fallbacks = ops = peak = 0
+        fwd_s = 0.0
         t0 = time.perf_counter()
         for k in range(args.steps):
             rs = step_reads(args.warmup + k, args.batch_reads)
             ts = time.perf_counter()
-            eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds, model), use_graphs=True)
+            order = plan_order(rs, ds, model)
+            if args.score_only:
+                eng.score_batch(rs, slots=args.slots, order=order, use_graphs=True)
+                fwd_s += time.perf_counter() - ts
+            else:
+                paths = eng.traceback_batch(rs, k=args.k, slots=args.slots, order=order, use_graphs=True)
+                fwd_s += eng.last_stats["forward_ms"] / 1e3
+                ops += sum(len(p.ops) for p in paths)
+            st = eng.last_stats
             cells += int(rs.lengths().sum()) * V
-            log(f"[bench] batched step {k}: {len(rs)} reads, {eng.last_stats['rows']} ticks, "
-                f"{time.perf_counter() - ts:.1f}s")
+            ticks += st["rows"]
+            fallbacks += st["walk_fallbacks"]
+            peak = max(peak, st["peak_device_bytes"])
+            log(f"[bench] batched step {k}: {len(rs)} reads, {st['rows']} ticks, {time.perf_counter() - ts:.1f}s")
         dt = time.perf_counter() - t0
         batch_gcups = cells / dt / 1e9
+        score_gcups = cells / fwd_s / 1e9
         result["batch"] = {"gcups": round(batch_gcups, 3), "ms_per_step": round(dt * 1e3 / args.steps, 2),
-                           "reads_per_step": args.batch_reads, "slots": args.slots}
-        log(f"[bench] batched: {batch_gcups:.2f} GCUPS ({dt:.2f}s)")
+                           "reads_per_step": args.batch_reads, "slots": args.slots,
+                           "work": "scores only" if args.score_only else
+                           f"scores and paths (checkpoint rows every {args.k} in HBM)",
+                           "score_only_gcups": round(score_gcups, 3), "ticks": ticks}
+        if not args.score_only:
+            result["batch"].update({"path_ops": ops, "walk_fallbacks": fallbacks,
+                                    "peak_device_gb": round(peak / 1e9, 1)})
+        log(f"[bench] batched: {batch_gcups:.2f} GCUPS ({dt:.2f}s); forward pass alone {score_gcups:.2f} GCUPS")
 
     # ---- roofline of the dominant kernel (profiled, separate short run) ----
     # HIP events bracket every row-kernel launch on the engine's own stream.
@@ -246,29 +275,16 @@ def main() -> int:
     log(f"[bench] k_brow_tile: {b_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
         f"k_row_tile: {p_ms * 1e3:.1f} us/launch")
 
-    # ---- optional: checkpointed traceback (paths), batched and per read ----
+    # ---- optional: the per-read checkpointed traceback, for comparison ----
     if args.traceback > 0:
         rs = step_reads(0, args.traceback)
-        cells = int(rs.lengths().sum()) * V
-        t0 = time.perf_counter()
-        paths = eng.traceback_batch(rs, k=500, order=plan_order(rs, ds, model), use_graphs=True)
-        dt = time.perf_counter() - t0
-        tst = dict(eng.last_stats)
-        sc = eng.score_batch(rs)
-        ok = all(p.score == int(sc[i]) for i, p in enumerate(paths))
-        n1 = min(2, len(rs))
         t1 = time.perf_counter()
-        for i in range(n1):
-            eng.align_read_with_traceback(rs[i], k=500)
+        for i in range(len(rs)):
+            eng.align_read_with_traceback(rs[i], k=args.k)
         dt1 = time.perf_counter() - t1
-        cells1 = int(rs.lengths()[:n1].sum()) * V
-        result["traceback"] = {"reads": len(rs), "k": 500, "batched_gcups": round(cells / dt / 1e9, 3),
-                               "batched_s": round(dt, 2), "per_read_gcups": round(cells1 / dt1 / 1e9, 3),
-                               "scores_match": ok, "ticks": tst.get("rows"),
-                               "walk_fallbacks": tst.get("walk_fallbacks"),
-                               "peak_device_gb": round(tst.get("peak_device_bytes", 0) / 1e9, 1)}
-        log(f"[bench] traceback: batched {cells / dt / 1e9:.2f} GCUPS ({len(rs)} reads, {dt:.1f}s), "
-            f"per read {cells1 / dt1 / 1e9:.2f} GCUPS, scores match: {ok}")
+        cells1 = int(rs.lengths().sum()) * V
+        result["traceback_per_read"] = {"reads": len(rs), "k": args.k, "gcups": round(cells1 / dt1 / 1e9, 3)}
+        log(f"[bench] per-read traceback: {cells1 / dt1 / 1e9:.2f} GCUPS ({len(rs)} reads)")
 
     cpu = None
     if not args.no_cpu:
@@ -283,17 +299,20 @@ def main() -> int:
         "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded graph and reads)",
         "config": {"workload": f"{wl.name}: 20 Mbp seeded DAG (SNP 1/25, indels 1-50 at 1/200), V={V} chars, "
                                f"n_s={info['n_s']}; reads log-normal median 3 kbp (1-30 kbp), 2% error, 1% N, "
-                               f"mismatchN, score-only",
+                               f"mismatchN, " + ("scores only" if args.score_only else
+                                                 f"scores and paths, checkpoint rows every {args.k} in HBM"),
                    "batch_reads": args.batch_reads, "plain_reads": args.plain_reads, "slots": args.slots,
                    "scheduler": scheduler, "schedule": result.get("schedule")},
         "autotune": result.get("autotune"),
         "plain_gcups": None if plain_gcups is None else round(plain_gcups, 3),
         "batched_gcups": None if batch_gcups is None else round(batch_gcups, 3),
+        "score_only_gcups": None if score_gcups is None else round(score_gcups, 3),
         "speedup_batched_vs_plain": (round(batch_gcups / plain_gcups, 2) if batch_gcups and plain_gcups else None),
+        "batch": result.get("batch"),
         "roofline": roofline, "cpu_baseline": cpu,
     }
-    if "traceback" in result:
-        line["traceback"] = result["traceback"]
+    if "traceback_per_read" in result:
+        line["traceback_per_read"] = result["traceback_per_read"]
     print(json.dumps(line), flush=True)
     return 0
 

@@ -70,7 +70,7 @@ struct chsga_ctx {
   int64_t tile_chars = 2040;  // one 256 x 8 block round even when c0 is not 8-aligned
   int64_t batch_tile_chars = 128;   // batched schedule (chsga_batch_schedule)
   int64_t batch_big_level = 16384;
-  int32_t batch_big_vpw = 4;
+  int32_t batch_big_vpw = 8;
   std::vector<uint32_t> h_lvl_off, h_up_off, h_dn_off;
   // host copies for traceback
   std::string h_labels;

@@ -43,7 +43,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("wall_ms", C.c_double), ("row_kernel_ms", C.c_double), ("row_kernel_launches", C.c_int64),
         ("sweep_ms", C.c_double), ("cells", C.c_int64), ("kernel_launches", C.c_int64), ("rows", C.c_int64),
-        ("walk_fallbacks", C.c_int64), ("peak_device_bytes", C.c_int64),
+        ("walk_fallbacks", C.c_int64), ("peak_device_bytes", C.c_int64), ("forward_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -121,6 +121,7 @@ typedef struct chsga_stats {
   int64_t rows;            /* DP rows computed (forward + recompute)        */
   int64_t walk_fallbacks;  /* traceback rows that needed the staged walk    */
   int64_t peak_device_bytes;
+  double forward_ms;       /* chsga_traceback_batch: wall clock of its forward passes (scores) */
 } chsga_stats;
 
 /* Context options, chsga_ctx_set_option. */
