@@ -23,6 +23,17 @@ void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void HostMem::alloc(size_t bytes) {
+  release();
+  if (bytes == 0) return;
+  hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+  n = bytes;
+}
+void HostMem::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  n = 0;
+}
 void DevMem::alloc(size_t bytes) {
   release();
   if (bytes == 0) return;
@@ -507,14 +518,14 @@ int chsga_ctx::forward(const char* read, int64_t len, int64_t k, int32_t** ckpt_
   int cur = 0;
   launch_init(R[cur], uint8_t(read[0]));
   if (ckpt_rows)
-    hip_check(hipMemcpyAsync(ckpt_rows[0], R[cur], sizeof(int32_t) * V, hipMemcpyDeviceToDevice, stream), "ckpt");
+    hip_check(hipMemcpyAsync(ckpt_rows[0], R[cur], sizeof(int32_t) * V, hipMemcpyDefault, stream), "ckpt");
   for (int64_t i = 1; i < len; ++i) {
     if (profile && ev_used + 3 > ev_pool.size()) flush_profile();
     launch_row(R[cur], R[cur ^ 1], uint8_t(read[i]));
     cur ^= 1;
     if (ckpt_rows && i % k == 0)
-      hip_check(hipMemcpyAsync(ckpt_rows[i / k], R[cur], sizeof(int32_t) * V, hipMemcpyDeviceToDevice, stream),
-                "ckpt");
+      hip_check(hipMemcpyAsync(ckpt_rows[i / k], R[cur], sizeof(int32_t) * V, hipMemcpyDefault, stream),
+                "ckpt");  // device or pinned host row
   }
   rows += len;
   return cur;
@@ -640,6 +651,12 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         break;
       case CHSGA_OPT_BATCHED_BACKWARD:
         ctx->batched_backward = value != 0;
+        break;
+      case CHSGA_OPT_CKPT_HBM_ROWS:
+        ctx->ckpt_hbm_rows = value < 0 ? -1 : value;
+        break;
+      case CHSGA_OPT_SEG_ROWS:
+        ctx->seg_rows_cap = value < 0 ? -1 : value;
         break;
       default:
         throw std::invalid_argument("unknown option");

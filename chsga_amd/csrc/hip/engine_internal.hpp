@@ -21,6 +21,18 @@ struct HipError : std::runtime_error {
 void set_error(const std::string& m);
 void hip_check(hipError_t e, const char* what);
 
+// Pinned host memory (hipHostMalloc), released with the owner.
+struct HostMem {
+  void* p = nullptr;
+  size_t n = 0;
+  HostMem() = default;
+  HostMem(const HostMem&) = delete;
+  HostMem& operator=(const HostMem&) = delete;
+  ~HostMem() { release(); }
+  void alloc(size_t bytes);
+  void release();
+};
+
 struct DevMem {
   void* p = nullptr;
   size_t n = 0;
@@ -99,6 +111,8 @@ struct chsga_ctx {
   // options and counters
   bool profile = false, use_graphs = false, force_staged = false;
   bool batched_backward = true;  // batched traceback: lock-step segment recompute when it fits
+  int64_t ckpt_hbm_rows = -1;     // per-read traceback: checkpoint rows kept in HBM (-1: all that fit)
+  int64_t seg_rows_cap = -1;      // per-read traceback: segment store rows (-1: by free memory)
   bool compact = false;  // char graph is the expansion of the string graph
   bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)
   int64_t launches = 0, rows = 0;

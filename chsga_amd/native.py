@@ -19,6 +19,7 @@ ALPHABET_MATCH_N = 4
 ALPHABET_MISMATCH_N = 5
 OP_DELETE, OP_MATCHSUB, OP_INSERT = 0, 1, 2
 OPT_PROFILE, OPT_USE_GRAPHS, OPT_STAGED_WALK, OPT_NARROW, OPT_BATCHED_BACKWARD = 1, 2, 5, 6, 7
+OPT_CKPT_HBM_ROWS, OPT_SEG_ROWS = 8, 9
 
 u64p = C.POINTER(C.c_uint64)
 u32p = C.POINTER(C.c_uint32)
@@ -44,6 +45,7 @@ class Stats(C.Structure):
         ("wall_ms", C.c_double), ("row_kernel_ms", C.c_double), ("row_kernel_launches", C.c_int64),
         ("sweep_ms", C.c_double), ("cells", C.c_int64), ("kernel_launches", C.c_int64), ("rows", C.c_int64),
         ("walk_fallbacks", C.c_int64), ("peak_device_bytes", C.c_int64), ("forward_ms", C.c_double),
+        ("host_bytes", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

@@ -122,6 +122,7 @@ typedef struct chsga_stats {
   int64_t walk_fallbacks;  /* traceback rows that needed the staged walk    */
   int64_t peak_device_bytes;
   double forward_ms;       /* chsga_traceback_batch: wall clock of its forward passes (scores) */
+  int64_t host_bytes;      /* chsga_traceback_read: checkpoint bytes in pinned host memory */
 } chsga_stats;
 
 /* Context options, chsga_ctx_set_option. */
@@ -131,6 +132,10 @@ typedef struct chsga_stats {
 #define CHSGA_OPT_NARROW 6       /* 1 (default): store scores as int16 when provably < 2^15; 0: int32 */
 #define CHSGA_OPT_BATCHED_BACKWARD 7 /* 1 (default): chsga_traceback_batch recomputes segments of 64 reads
                                         at once when its row buffers fit; 0: read by read */
+#define CHSGA_OPT_CKPT_HBM_ROWS 8 /* chsga_traceback_read: at most this many checkpoint rows in HBM, the
+                                     rest in pinned host memory (-1, default: as many as fit) */
+#define CHSGA_OPT_SEG_ROWS 9      /* per-read segment store: whole segments up to this many rows, else
+                                     two levels (-1, default: whole when it fits in free memory) */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);

@@ -204,3 +204,62 @@ def test_batched_backward_staged_walk_gives_valid_paths(tmp_path):
             rc, bad = O.validate_path(d, reads[i], tb.ops, tb.score)
             assert rc == 0, (t, i, rc, bad)
         eng.close()
+
+
+@pytest.mark.parametrize("k", [7, 40])
+def test_checkpoints_in_host_memory_and_two_level_segments(k):
+    """The per-read traceback with checkpoint rows in pinned host memory
+    (one row kept in HBM) and with the two-level segment store (snapshots
+    every ceil(sqrt(k)) rows, sub-segments of that many rows) gives the same
+    paths as with everything in HBM; so does the batched traceback's
+    read-by-read backward with two-level segments."""
+    ds, rs = small_workload(seed=46, n_reads=6, length=150, error=0.08, n_rate=0.02)
+    d = ds.desc()
+    eng = Engine(ds)
+    ref = [eng.align_read_with_traceback(rs[i], k=k) for i in range(len(rs))]
+    assert eng.last_stats["host_bytes"] == 0
+    eng.set_option(N.OPT_CKPT_HBM_ROWS, 1)
+    print("host checkpoints", k, flush=True)
+    host = [eng.align_read_with_traceback(rs[i], k=k) for i in range(len(rs))]
+    assert eng.last_stats["host_bytes"] > 0
+    eng.set_option(N.OPT_CKPT_HBM_ROWS, -1)
+    eng.set_option(N.OPT_SEG_ROWS, 4)
+    print("two-level segments", k, flush=True)
+    two = [eng.align_read_with_traceback(rs[i], k=k) for i in range(len(rs))]
+    eng.set_option(N.OPT_BATCHED_BACKWARD, 0)
+    print("batched forward, read-by-read two-level backward", k, flush=True)
+    bat = eng.traceback_batch(rs, k=k)
+    for i in range(len(rs)):
+        for other in (host[i], two[i], bat[i]):
+            assert other.score == ref[i].score and np.array_equal(other.ops, ref[i].ops), (k, i)
+        assert O.validate_path(d, rs[i], ref[i].ops, ref[i].score)[0] == 0
+    eng.close()
+
+
+def test_c5_traceback_validates():
+    """Traceback on the largest workload's graph (116.7M chars, structural
+    bubbles, matchN, k = 500): the batched traceback falls back to the
+    read-by-read backward (64-slot row buffers do not fit there); checkpoint
+    rows in pinned host memory with the two-level segment store give the same
+    path; every path validates and rescores to the batched score."""
+    w = WORKLOADS["c5"]
+    ds = w.dataset()
+    rs = w.readset(ds, n_reads=4)
+    clean = ReadSet.generate(ds, seed=9, n_reads=1, length=300)
+    reads = ReadSet.from_list([rs[0][:600], rs[1][:400], clean[0]])
+    d = ds.desc(w.alphabet)
+    eng = Engine(ds, w.alphabet)
+    print("c5 batched traceback", reads.lengths().tolist(), flush=True)
+    out = eng.traceback_batch(reads, k=500)
+    sc = eng.score_batch(reads)
+    for i, tb in enumerate(out):
+        assert tb.score == sc[i]
+        assert O.validate_path(d, reads[i], tb.ops, tb.score)[0] == 0
+    assert out[2].score == 0
+    eng.set_option(N.OPT_CKPT_HBM_ROWS, 0)
+    eng.set_option(N.OPT_SEG_ROWS, 64)
+    print("c5 per-read traceback, host checkpoints, two-level segments", flush=True)
+    tb = eng.align_read_with_traceback(reads[0], k=500)
+    assert eng.last_stats["host_bytes"] > 0
+    assert tb.score == out[0].score and np.array_equal(tb.ops, out[0].ops)
+    eng.close()
