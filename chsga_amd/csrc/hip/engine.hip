@@ -369,30 +369,43 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     std::vector<uint32_t> vfirst(n_s + kSweepPad, 0u);
     for (int64_t k = 0; k < n_s; ++k) vfirst[k] = c->h_str_off[sw_vert[k]];
     upload(c->d_vfirst, vfirst, s);
-    // levels whose lists hold at most two pulls of weight < 2^16: one 16-byte
-    // record per vertex {first, src0, src1, w0 | w1 << 16}; a missing pull
-    // is the vertex's own first char with weight 0
-    auto pairs = [&](const std::vector<uint32_t>& off, const std::vector<int2>& pin,
-                     const std::vector<uint32_t>& lmax, std::vector<uint8_t>& ok) {
-      std::vector<uint4> rec(n_s + kSweepPad, make_uint4(0, 0, 0, 0));
-      ok.assign(size_t(c->J), 0);
+    // levels whose lists hold at most R = 2, 4 or 8 pulls of weight < 2^16:
+    // one record of 4, 8 or 16 words per vertex, {first, src_0 .. src_R-1,
+    // w_0 | w_1 << 16, ...}; a missing pull is the vertex's own first char at
+    // weight 0.  Level j's records start at word rbase[j]; zero records pad
+    // each level by the widest wave, so a wave reads only in-bounds chars.
+    auto records = [&](const std::vector<uint32_t>& off, const std::vector<int2>& pin,
+                       const std::vector<uint32_t>& lmax, std::vector<uint8_t>& R, std::vector<uint64_t>& rbase) {
+      std::vector<uint32_t> rec;
+      R.assign(size_t(c->J), 0);
+      rbase.assign(size_t(c->J), 0);
       for (int32_t j = 0; j < c->J; ++j) {
-        bool fits = lmax[j] <= 2;
+        const int r = lmax[j] <= 2 ? 2 : lmax[j] <= 4 ? 4 : lmax[j] <= 8 ? 8 : 0;
+        bool fits = r > 0;
         for (uint32_t e = off[c->h_lvl_off[j]]; fits && e < off[c->h_lvl_off[j + 1]]; ++e)
           fits = pin[e].y >= 0 && pin[e].y < 65536;
-        ok[j] = fits;
         if (!fits) continue;
+        const int W = sweep_record_words(r);
+        R[j] = uint8_t(r);
+        rbase[j] = rec.size();
+        const uint32_t nv = c->h_lvl_off[j + 1] - c->h_lvl_off[j];
+        rec.resize(rec.size() + size_t(nv + kSweepPad) * W, 0u);
         for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) {
+          uint32_t* w = rec.data() + rbase[j] + size_t(k - c->h_lvl_off[j]) * W;
           const uint32_t n = off[k + 1] - off[k], f = vfirst[k];
-          const uint32_t s0 = n > 0 ? uint32_t(pin[off[k]].x) : f, s1 = n > 1 ? uint32_t(pin[off[k] + 1].x) : f;
-          const uint32_t w0 = n > 0 ? uint32_t(pin[off[k]].y) : 0u, w1 = n > 1 ? uint32_t(pin[off[k] + 1].y) : 0u;
-          rec[k] = make_uint4(f, s0, s1, w0 | (w1 << 16));
+          w[0] = f;
+          for (int q = 0; q < r; ++q) {
+            const bool has = uint32_t(q) < n;
+            w[1 + q] = has ? uint32_t(pin[off[k] + q].x) : f;
+            w[1 + r + q / 2] |= (has ? uint32_t(pin[off[k] + q].y) : 0u) << (16 * (q & 1));
+          }
         }
       }
+      if (rec.empty()) rec.assign(4, 0u);
       return rec;
     };
-    upload(c->d_bup_rec, pairs(up_off, bup, c->h_up_maxp, c->h_up_pair), s);
-    upload(c->d_bdn_rec, pairs(dn_off, bdn, c->h_dn_maxp, c->h_dn_pair), s);
+    upload(c->d_bup_rec, records(up_off, bup, c->h_up_maxp, c->h_up_rec, c->h_up_rbase), s);
+    upload(c->d_bdn_rec, records(dn_off, bdn, c->h_dn_maxp, c->h_dn_rec, c->h_dn_rbase), s);
   }
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);

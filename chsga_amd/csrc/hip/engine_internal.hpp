@@ -55,6 +55,9 @@ constexpr int64_t kSweepPad = 8;
 // largest number of pull entries the batched level sweep loads per vertex at
 // once (one vertex per wave, on levels whose lists exceed two entries)
 constexpr int kSweepChunk = 16;
+// words of a batched sweep pull record with R pull slots: first char, R
+// sources, R 16-bit weights, rounded up to a power of two
+constexpr int sweep_record_words(int R) { return R == 2 ? 4 : R == 4 ? 8 : 16; }
 // chars the batched row buffers hold past V (>= one row-pass chunk)
 constexpr int64_t kRowPad = 32;
 struct TopSet {
@@ -102,9 +105,12 @@ struct chsga_ctx {
   // lists with the last char of each source string
   chsga_engine::DevMem d_vfirst, d_bup_in, d_bdn_in;
   std::vector<uint32_t> h_up_maxp, h_dn_maxp;  // longest pull list of each level
-  // per vertex {first, src0, src1, w0 | w1 << 16} on levels flagged in h_*_pair
+  // pull records {first, src_0 .. src_R-1, packed 16-bit weights} of levels
+  // with h_*_rec[j] = R in {2, 4, 8} (0: the level takes the pull lists),
+  // level j's at word h_*_rbase[j]
   chsga_engine::DevMem d_bup_rec, d_bdn_rec;
-  std::vector<uint8_t> h_up_pair, h_dn_pair;
+  std::vector<uint8_t> h_up_rec, h_dn_rec;
+  std::vector<uint64_t> h_up_rbase, h_dn_rbase;
   TopSet top_plain;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
