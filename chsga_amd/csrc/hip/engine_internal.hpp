@@ -51,7 +51,7 @@ struct chsga_batch_state;
 // Top levels of the CH handled by one LDS-resident workgroup.
 constexpr int64_t kTopPlain = 4096;  // string vertices: 2 x 4 B each in LDS
 // entries the batched level sweep may read past a table's end (>= its vertices per wave)
-constexpr int64_t kSweepPad = 8;
+constexpr int64_t kSweepPad = 16;
 // largest number of pull entries the batched level sweep loads per vertex at
 // once (one vertex per wave, on levels whose lists exceed two entries)
 constexpr int kSweepChunk = 16;

@@ -178,7 +178,7 @@ int chsga_score_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* l
 typedef struct chsga_batch_schedule {
   int32_t tile_chars; /* chars per row-pass tile, whole strings (32..4096); default 128 */
   int32_t big_level;  /* CH levels with at least this many string vertices ... */
-  int32_t big_vpw;    /* ... take this many vertices per wave (1, 2, 4 or 8); smaller levels take 1 */
+  int32_t big_vpw;    /* ... take this many vertices per wave (1, 2, 4, 8 or 16); smaller levels take 1 */
 } chsga_batch_schedule;
 int chsga_get_batch_schedule(chsga_ctx* ctx, chsga_batch_schedule* out);
 int chsga_set_batch_schedule(chsga_ctx* ctx, const chsga_batch_schedule* sched);
