@@ -48,7 +48,8 @@ void DevMem::release() {
 
 template <class T>
 static void upload(DevMem& d, const std::vector<T>& h, hipStream_t s) {
-  d.alloc(std::max<size_t>(1, h.size()) * sizeof(T));
+  const size_t bytes = std::max<size_t>(1, h.size()) * sizeof(T);
+  if (d.n != bytes) d.alloc(bytes);  // same size: reuse the allocation
   if (h.empty()) return;
   hip_check(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
   // the source is pageable and may be a temporary: wait until it has been read
