@@ -315,6 +315,28 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
       upload(c->d_lclass, lclass, s);
       upload(c->d_bnb, bnb, s);
     }
+    // packed rows (2-bit deltas, batch_packed.hip.inc): exact for unit costs;
+    // their scan does more per char, so longer tiles (fewer chunk starts) pay
+    c->packed = c->batch_fast && c->allow_packed && c->msc == 0 && c->ssc == 1 && c->dsc == 1 && c->isc == 1;
+    if (c->packed) c->batch_tile_chars = 512;
+    if (c->packed) {
+      // per string: {string, length - 1} of its at most two in-neighbour strings
+      std::vector<int4> pbnb(n_s + 32, make_int4(-1, -1, -1, -1));
+      for (int64_t st = 0; st < n_s; ++st) {
+        const int4 e = bnb[st];
+        int32_t v[4] = {-1, 0, -1, 0};
+        for (int q = 0; q < 2; ++q) {
+          const int32_t l = q ? e.z : e.x;
+          if (l < 0) continue;
+          const int64_t x = int64_t(std::upper_bound(c->h_str_off.begin(), c->h_str_off.end(), uint32_t(l)) -
+                                    c->h_str_off.begin()) - 1;
+          v[2 * q] = int32_t(x);
+          v[2 * q + 1] = int32_t(c->h_str_off[x + 1] - c->h_str_off[x]) - 1;
+        }
+        pbnb[st] = make_int4(v[0], v[1], v[2], v[3]);
+      }
+      upload(c->d_pbnb, pbnb, s);
+    }
   }
   if (!c->batch_fast) {
     // general batched kernel: first char of every char's string
@@ -358,17 +380,23 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
       return all;
     };
     const uint32_t pad = std::max(maxp(up_off, c->h_up_maxp), maxp(dn_off, c->h_dn_maxp)) + 2 * kSweepChunk;
+    // a pull reads the last char of its source string x (char rows) or, in
+    // packed rows, plane PL of x (plane F when x has one char); a vertex's
+    // own value is its first char (plane F)
+    auto pull_src = [&](uint32_t x) -> int32_t {
+      if (!c->packed) return int32_t(c->h_str_off[x + 1] - 1);
+      return int32_t(3 * x + (c->h_str_off[x + 1] - c->h_str_off[x] > 1 ? 1 : 0));
+    };
     auto by_last = [&](const std::vector<int2>& in) {
       std::vector<int2> out(in.size() + pad, make_int2(0, 0));
-      for (size_t e = 0; e < in.size(); ++e)
-        out[e] = make_int2(int32_t(c->h_str_off[(uint32_t(in[e].x) & 0x7fffffffu) + 1] - 1), in[e].y);
+      for (size_t e = 0; e < in.size(); ++e) out[e] = make_int2(pull_src(uint32_t(in[e].x) & 0x7fffffffu), in[e].y);
       return out;
     };
     const std::vector<int2> bup = by_last(up_in), bdn = by_last(dn_in);
     upload(c->d_bup_in, bup, s);
     upload(c->d_bdn_in, bdn, s);
     std::vector<uint32_t> vfirst(n_s + kSweepPad, 0u);
-    for (int64_t k = 0; k < n_s; ++k) vfirst[k] = c->h_str_off[sw_vert[k]];
+    for (int64_t k = 0; k < n_s; ++k) vfirst[k] = c->packed ? 3 * sw_vert[k] : c->h_str_off[sw_vert[k]];
     upload(c->d_vfirst, vfirst, s);
     // levels whose lists hold at most R = 2, 4 or 8 pulls of weight < 2^16:
     // one record of 4, 8 or 16 words per vertex, {first, src_0 .. src_R-1,
@@ -614,6 +642,10 @@ int chsga_mem_info(chsga_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
 }
 
 int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out) {
+  return chsga_ctx_create_ex(g, device, 0u, out);
+}
+
+int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, chsga_ctx** out) {
   if (!out) {
     set_error("null output pointer");
     return CHSGA_EINVAL;
@@ -621,9 +653,11 @@ int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out) {
   *out = nullptr;
   chsga_ctx* c = nullptr;
   ENGINE_GUARD({
+    if (flags & ~uint32_t(CHSGA_CREATE_CHAR_ROWS)) throw std::invalid_argument("unknown context flags");
     c = new chsga_ctx();
     try {
       c->device = device;
+      c->allow_packed = (flags & CHSGA_CREATE_CHAR_ROWS) == 0;
       build_context(c, g);
     } catch (...) {
       delete c;
@@ -671,6 +705,11 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         break;
       case CHSGA_OPT_SEG_ROWS:
         ctx->seg_rows_cap = value < 0 ? -1 : value;
+        break;
+      case CHSGA_OPT_PACKED_ROWS:
+        if ((value != 0) != ctx->packed)
+          throw std::invalid_argument(ctx->packed ? "the batched rows are packed"
+                                                  : "the batched rows are not packed (chsga_ctx_create_ex decides)");
         break;
       default:
         throw std::invalid_argument("unknown option");

@@ -97,6 +97,10 @@ struct chsga_ctx {
   chsga_engine::DevMem d_labels, d_in_off, d_in_src, d_first_bits, d_tile_char, d_tile_str, d_str_first;
   bool nb2_all = false;  // compact, and no first char has more than 2 in-neighbours
   bool batch_fast = false;  // nb2_all and every label in ACGTN: the batched compact row kernel
+  bool allow_packed = true;  // chsga_ctx_create_ex without CHSGA_CREATE_CHAR_ROWS
+  bool packed = false;       // batched rows packed as 2-bit deltas (batch_fast, unit costs)
+  chsga_engine::DevMem d_pbnb;  // packed: per string {string, length - 1} of <= 2 in-neighbour strings
+  chsga_engine::DevMem d_cstr;  // packed: string of each char (batched traceback walk)
   chsga_engine::DevMem d_lclass;  // match class of each char's label (batched compact kernel)
   chsga_engine::DevMem d_bnb;     // per string: {last, first} chars of <= 2 in-neighbour strings
   chsga_engine::DevMem d_cfirst;  // first char of each char's string (general batched kernel)

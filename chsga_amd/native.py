@@ -19,7 +19,8 @@ ALPHABET_MATCH_N = 4
 ALPHABET_MISMATCH_N = 5
 OP_DELETE, OP_MATCHSUB, OP_INSERT = 0, 1, 2
 OPT_PROFILE, OPT_USE_GRAPHS, OPT_STAGED_WALK, OPT_NARROW, OPT_BATCHED_BACKWARD = 1, 2, 5, 6, 7
-OPT_CKPT_HBM_ROWS, OPT_SEG_ROWS = 8, 9
+OPT_CKPT_HBM_ROWS, OPT_SEG_ROWS, OPT_PACKED_ROWS = 8, 9, 10
+CREATE_CHAR_ROWS = 1
 
 u64p = C.POINTER(C.c_uint64)
 u32p = C.POINTER(C.c_uint32)
@@ -104,6 +105,7 @@ ENGINE_SYMBOLS = {
     "chsga_last_error": (C.c_char_p, []),
     "chsga_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "chsga_ctx_create": (C.c_int, [C.POINTER(GraphDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "chsga_ctx_create_ex": (C.c_int, [C.POINTER(GraphDesc), C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
     "chsga_ctx_destroy": (None, [C.c_void_p]),
     "chsga_mem_info": (C.c_int, [C.c_void_p, i64p, i64p]),
     "chsga_ctx_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),

@@ -136,6 +136,8 @@ typedef struct chsga_stats {
                                      rest in pinned host memory (-1, default: as many as fit) */
 #define CHSGA_OPT_SEG_ROWS 9      /* per-read segment store: whole segments up to this many rows, else
                                      two levels (-1, default: whole when it fits in free memory) */
+#define CHSGA_OPT_PACKED_ROWS 10  /* read-only check: 1 if the batched rows are packed (setting a different
+                                     value is an error: the choice is made by chsga_ctx_create_ex) */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);
@@ -146,6 +148,13 @@ int chsga_device_count(int* count);
 int chsga_mem_info(chsga_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out);
+/* chsga_ctx_create with flags; no counterpart in the reference.
+ * CHSGA_CREATE_CHAR_ROWS: the batched path stores one score per char and
+ * slot even where packed rows (2-bit deltas per char, exact under unit
+ * costs on compact graphs, the default there) apply.  Results are the same
+ * either way. */
+#define CHSGA_CREATE_CHAR_ROWS 1u
+int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, chsga_ctx** out);
 void chsga_ctx_destroy(chsga_ctx* ctx);
 int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value);
 

@@ -304,3 +304,39 @@ def test_int16_storage_equals_int32_storage():
     eng.set_option(N.OPT_NARROW, 1)
     assert np.array_equal(eng.align_combo(long_read), lref)
     eng.close()
+
+
+def test_packed_rows_equal_char_rows():
+    """Batched rows packed as 2-bit deltas (the default on compact graphs
+    with unit costs) give the scores and paths of one score per char, on c1,
+    on small graphs with N and on reads of uneven length; a graph with other
+    costs keeps char rows."""
+    from chsga_amd import WORKLOADS
+    w = WORKLOADS["c1"]
+    cases = [(w.dataset(), w.alphabet, None)]
+    ds2, rs2 = small_workload(seed=51, n_reads=90, length=150, error=0.08, n_rate=0.03)
+    cases.append((ds2, N.ALPHABET_MISMATCH_N, rs2))
+    cases.append((ds2, N.ALPHABET_MATCH_N, rs2))
+    for ds, alpha, rs in cases:
+        if rs is None:
+            rs = w.readset(ds, n_reads=40)
+        rs = ReadSet.from_list([rs[i][: 1 + (i * 29) % len(rs[i])] for i in range(len(rs))])
+        a = Engine(ds, alpha)
+        b = Engine(ds, alpha, packed=False)
+        a.set_option(N.OPT_PACKED_ROWS, 1)
+        b.set_option(N.OPT_PACKED_ROWS, 0)
+        _say("packed vs char rows", alpha, len(rs))
+        sa, sb = a.score_batch(rs), b.score_batch(rs)
+        assert np.array_equal(sa, sb)
+        assert np.array_equal(sa, a.align_combo(rs))
+        ta, tb = a.traceback_batch(rs, k=40), b.traceback_batch(rs, k=40)
+        for i in range(len(rs)):
+            assert ta[i].score == tb[i].score and np.array_equal(ta[i].ops, tb[i].ops), i
+        a.close()
+        b.close()
+    d = ds2.desc()
+    d.del_score = 2
+    e = Engine(ds2, desc=d)
+    with pytest.raises(N.NativeError):
+        e.set_option(N.OPT_PACKED_ROWS, 1)
+    e.close()
