@@ -65,20 +65,49 @@ def batch_kernel_bytes(info: dict, score_bytes: int, slots: int = 64) -> int:
     return 2 * slots * score_bytes * V + V + V // 8 + 16 * n_s
 
 
-def measured_traffic(bytes_per_launch: int):
-    """HBM bytes per k_brow_tile launch from the committed counter summary,
-    or None when it was taken on a different workload."""
-    p = ROOT / "profiles" / "k_brow_tile_c3_hbm_traffic.md"
+def packed_kernel_bytes(ds, tile_chars: int, score_bytes: int, slots: int = 64) -> int:
+    """Algorithmic (compulsory) bytes of one k_brow_packed launch (packed
+    rows: 2-bit deltas per char and slot, F/PL/PF planes per string).  Read
+    and written once each: the delta words of every tile (whole 16-char
+    groups per tile, 4 bytes per slot), plane F of every string and planes
+    PF and PL of every string of more than one char (score bytes per slot);
+    read once: one label class byte and one first-char bit per char and the
+    16-byte in-neighbour entry of every string.  Gathered in-neighbour planes
+    are the same planes of the input row."""
+    a = ds.arrays()
+    lens = np.diff(a["str_off"].astype(np.int64))
+    groups, acc = 0, 0
+    for ln in lens:  # the engine's tiling: whole strings, at most tile_chars chars
+        if acc > 0 and acc + ln > tile_chars:
+            groups += (acc + 15) // 16
+            acc = 0
+        acc += int(ln)
+    groups += (acc + 15) // 16
+    n_s, n_multi = len(lens), int((lens > 1).sum())
+    V = int(a["V"])
+    return 2 * groups * slots * 4 + 2 * (n_s + 2 * n_multi) * slots * score_bytes + V + V // 8 + 16 * n_s
+
+
+def profile_summary(kernel: str, bytes_per_launch: int) -> dict:
+    """Figures of the committed counter summary of `kernel` on c3 (HBM
+    traffic, VALU instructions per launch), or {} when it was taken on
+    different algorithmic bytes per launch."""
+    p = ROOT / "profiles" / f"{kernel}_c3_hbm_traffic.md"
     if not p.exists():
-        return None
+        return {}
     vals = {}
     for line in p.read_text().splitlines():
         k, _, v = line.partition(":")
-        if k.strip() in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch") and v.strip().isdigit():
+        if v.strip().isdigit():
             vals[k.strip()] = int(v.strip())
     if vals.get("algorithmic_bytes_per_launch") != bytes_per_launch:
-        return None
-    return vals.get("traffic_bytes_per_launch")
+        return {}
+    return vals
+
+
+# VALU issue rate of MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per SIMD every 4 cycles (16 lanes)
+VALU_PEAK_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 def row_bytes_survey(info: dict) -> int:
@@ -258,12 +287,21 @@ def main() -> int:
     eng.set_option(N.OPT_PROFILE, 0)
     b_ms = stb["row_kernel_ms"] / max(1, stb["row_kernel_launches"])
     p_ms = stp["row_kernel_ms"] / max(1, stp["row_kernel_launches"])
-    kb = batch_kernel_bytes(info, score_bytes, args.slots)
+    packed = eng.packed
+    kernel = "k_brow_packed" if packed else "k_brow_tile"
+    kb = (packed_kernel_bytes(ds, eng.schedule()["tile_chars"], score_bytes, args.slots) if packed
+          else batch_kernel_bytes(info, score_bytes, args.slots))
     pb = row_kernel_bytes(info, score_bytes)
     achieved = kb / (b_ms * 1e-3) / 1e9
+    prof = profile_summary(kernel, kb)
+    valu = None
+    if prof.get("valu_insts_per_launch"):
+        at_peak = prof["valu_insts_per_launch"] / VALU_PEAK_INSTS
+        valu = {"insts_per_launch": prof["valu_insts_per_launch"], "peak_insts_per_s": VALU_PEAK_INSTS,
+                "us_at_peak": round(at_peak * 1e6, 1), "frac": round(at_peak / (b_ms * 1e-3), 4)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kb),
-                "kernel": "k_brow_tile", "kernel_avg_us": round(b_ms * 1e3, 2),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("traffic_bytes_per_launch"),
+                "valu": valu, "kernel": kernel, "kernel_avg_us": round(b_ms * 1e3, 2),
                 "bytes_per_launch": kb, "launches_timed": stb["row_kernel_launches"],
                 "score_storage_bytes": score_bytes,
                 "plain_kernel": {"kernel": "k_row_tile", "avg_us": round(p_ms * 1e3, 2), "bytes_per_launch": pb,
@@ -272,7 +310,7 @@ def main() -> int:
                                                                     max(1, stp["row_kernel_launches"]), 2)},
                 "batched_sweep_post_avg_us_per_tick": round(stb["sweep_ms"] * 1e3 /
                                                             max(1, stb["row_kernel_launches"]), 2)}
-    log(f"[bench] k_brow_tile: {b_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
+    log(f"[bench] {kernel}: {b_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
         f"k_row_tile: {p_ms * 1e3:.1f} us/launch")
 
     # ---- optional: the per-read checkpointed traceback, for comparison ----

@@ -206,6 +206,11 @@ class Engine:
     def set_option(self, opt: int, value: int) -> None:
         N._check_engine(self._lib.chsga_ctx_set_option(self._h, opt, value))
 
+    @property
+    def packed(self) -> bool:
+        """True when the batched rows are packed (2-bit deltas per char)."""
+        return self._lib.chsga_ctx_set_option(self._h, N.OPT_PACKED_ROWS, 1) == 0
+
     # --- batched schedule and its autotuner ----------------------------------
     def schedule(self) -> dict:
         sc = N.BatchSchedule()

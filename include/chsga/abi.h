@@ -196,7 +196,8 @@ int chsga_set_batch_schedule(chsga_ctx* ctx, const chsga_batch_schedule* sched);
  * untimed ones) of candidate schedules on 64 slots filled from the given
  * reads (read chars are reused cyclically when a read is shorter than the
  * run) and keeps the fastest, by coordinate descent over tile_chars, then
- * big_vpw, then big_level.  results[max_results] receives every candidate
+ * big_vpw, then big_level; a candidate displaces the schedule kept so far
+ * only when it is more than 1% faster (timing noise).  results[max_results] receives every candidate
  * tried with its microseconds per tick; *n_results their number.  Scores are
  * not computed.  No counterpart in the reference. */
 typedef struct chsga_tune_result {
