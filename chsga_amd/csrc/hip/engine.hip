@@ -321,6 +321,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     if (c->packed) c->batch_tile_chars = 512;
     if (c->packed) {
       // per string: {string, length - 1} of its at most two in-neighbour strings
+      // (both -1: none)
       std::vector<int4> pbnb(n_s + 32, make_int4(-1, -1, -1, -1));
       for (int64_t st = 0; st < n_s; ++st) {
         const int4 e = bnb[st];
@@ -332,6 +333,16 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
                                     c->h_str_off.begin()) - 1;
           v[2 * q] = int32_t(x);
           v[2 * q + 1] = int32_t(c->h_str_off[x + 1] - c->h_str_off[x]) - 1;
+        }
+        // one in-neighbour: it fills both entries (the row pass takes the
+        // minimum over both)
+        if (v[0] < 0) {
+          v[0] = v[2];
+          v[1] = v[3];
+        }
+        if (v[2] < 0) {
+          v[2] = v[0];
+          v[3] = v[1];
         }
         pbnb[st] = make_int4(v[0], v[1], v[2], v[3]);
       }
