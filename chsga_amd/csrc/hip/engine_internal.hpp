@@ -101,6 +101,7 @@ struct chsga_ctx {
   bool packed = false;       // batched rows packed as 2-bit deltas (batch_fast, unit costs)
   chsga_engine::DevMem d_pbnb;  // packed: per string {string, length - 1} of <= 2 in-neighbour strings
   chsga_engine::DevMem d_cstr;  // packed: string of each char (batched traceback walk)
+  chsga_engine::DevMem d_lbits;  // packed: per 32 chars, label class bit planes 0-5 and first-char bits
   chsga_engine::DevMem d_lclass;  // match class of each char's label (batched compact kernel)
   chsga_engine::DevMem d_bnb;     // per string: {last, first} chars of <= 2 in-neighbour strings
   chsga_engine::DevMem d_cfirst;  // first char of each char's string (general batched kernel)
