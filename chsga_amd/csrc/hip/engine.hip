@@ -346,17 +346,9 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
         }
         pbnb[st] = make_int4(v[0], v[1], v[2], v[3]);
       }
-      upload(c->d_pbnb, pbnb, s);
-      // per 32 chars, 8 words: bit planes 0-5 of the label classes (bit p
-      // of plane k: bit k of p's class), plane 6 the first-char bits, plane 7
-      // zero; the packed row pass reads two such records per chunk with one
-      // scalar load
-      std::vector<uint32_t> lbits(size_t(8 * (V / 32 + 2)), 0u);
-      for (int64_t p = 0; p < V; ++p)
-        for (int k = 0; k < 6; ++k)
-          if ((lclass[p] >> k) & 1) lbits[size_t(8 * (p / 32) + k)] |= 1u << (p % 32);
-      for (int64_t w = 0; w < V / 32 + 2; ++w) lbits[size_t(8 * w + 6)] = first_bits[size_t(w)];
-      upload(c->d_lbits, lbits, s);
+      // the row pass's chunk table is built from these per tile schedule
+      c->h_pbnb = std::move(pbnb);
+      c->h_lclass = lclass;
     }
   }
   if (!c->batch_fast) {

@@ -99,9 +99,9 @@ struct chsga_ctx {
   bool batch_fast = false;  // nb2_all and every label in ACGTN: the batched compact row kernel
   bool allow_packed = true;  // chsga_ctx_create_ex without CHSGA_CREATE_CHAR_ROWS
   bool packed = false;       // batched rows packed as 2-bit deltas (batch_fast, unit costs)
-  chsga_engine::DevMem d_pbnb;  // packed: per string {string, length - 1} of <= 2 in-neighbour strings
+  std::vector<int4> h_pbnb;        // packed: per string {string, length - 1} of its two in-neighbour strings
+  std::vector<uint8_t> h_lclass;   // packed: match class of each char's label
   chsga_engine::DevMem d_cstr;  // packed: string of each char (batched traceback walk)
-  chsga_engine::DevMem d_lbits;  // packed: per 32 chars, label class bit planes 0-5 and first-char bits
   chsga_engine::DevMem d_lclass;  // match class of each char's label (batched compact kernel)
   chsga_engine::DevMem d_bnb;     // per string: {last, first} chars of <= 2 in-neighbour strings
   chsga_engine::DevMem d_cfirst;  // first char of each char's string (general batched kernel)
