@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>

@@ -175,9 +175,13 @@ def test_batched_backward_equals_per_read_backward(k):
     eng.set_option(N.OPT_NARROW, 0)
     print("batched backward, int32 rows", k, flush=True)
     c = eng.traceback_batch(rs, k=k)
+    eng.set_option(N.OPT_NARROW, 1)
+    print("batched backward, two slot groups and two backward workers", k, flush=True)
+    d = eng.traceback_batch(rs, k=k, slots=128)
     for i in range(len(rs)):
-        assert a[i].score == b[i].score == c[i].score
+        assert a[i].score == b[i].score == c[i].score == d[i].score
         assert np.array_equal(a[i].ops, b[i].ops) and np.array_equal(a[i].ops, c[i].ops), (k, i)
+        assert np.array_equal(a[i].ops, d[i].ops), (k, i)
     eng.close()
 
 
