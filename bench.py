@@ -65,27 +65,51 @@ def batch_kernel_bytes(info: dict, score_bytes: int, slots: int = 64) -> int:
     return 2 * slots * score_bytes * V + V + V // 8 + 16 * n_s
 
 
+def packed_layout(ds, tile_chars: int) -> tuple:
+    """(delta word groups, row-pass chunks) of packed rows under the engine's
+    tiling: whole strings per tile, at most tile_chars chars unless one
+    string is longer; each tile's deltas take whole 16-char groups; chunks
+    of at most 32 chars, cut before a tile's fifth string start within
+    them."""
+    a = ds.arrays()
+    so = [int(x) for x in a["str_off"]]
+    n_s = len(so) - 1
+    groups = chunks = 0
+    s = 0
+    while s < n_s:  # one tile: strings s .. e - 1
+        e, acc = s, 0
+        while e < n_s and (acc == 0 or acc + so[e + 1] - so[e] <= tile_chars):
+            acc += so[e + 1] - so[e]
+            e += 1
+        groups += (acc + 15) // 16
+        # chunks: string starts of the tile, relative to its first char
+        c0, c1 = so[s], so[e]
+        starts = [so[i] - c0 for i in range(s, e)]
+        p, i, n = 0, 0, c1 - c0
+        while p < n:
+            while i < len(starts) and starts[i] < p:
+                i += 1
+            cut = starts[i + 4] - p if i + 4 < len(starts) and starts[i + 4] - p < 64 else 64
+            p += min(32, cut, n - p)
+            chunks += 1
+        s = e
+    return groups, chunks
+
+
 def packed_kernel_bytes(ds, tile_chars: int, score_bytes: int, slots: int = 64) -> int:
     """Algorithmic (compulsory) bytes of one k_brow_packed launch (packed
-    rows: 2-bit deltas per char and slot, F/PL/PF planes per string).  Read
+    rows: 2-bit codes per char and slot, F/PL/PF planes per string).  Read
     and written once each: the delta words of every tile (whole 16-char
-    groups per tile, 4 bytes per slot), plane F of every string and planes
-    PF and PL of every string of more than one char (score bytes per slot);
-    read once: one label class byte and one first-char bit per char and the
-    16-byte in-neighbour entry of every string.  Gathered in-neighbour planes
-    are the same planes of the input row."""
+    groups per tile, 4 bytes per slot), planes F and PF of every string and
+    plane PL of every string of more than one char (score bytes per slot);
+    read once: the 128-byte table entry of every row-pass chunk (char count,
+    string starts, label classes, gathered planes).  Gathered in-neighbour
+    planes are the same planes of the input row."""
     a = ds.arrays()
     lens = np.diff(a["str_off"].astype(np.int64))
-    groups, acc = 0, 0
-    for ln in lens:  # the engine's tiling: whole strings, at most tile_chars chars
-        if acc > 0 and acc + ln > tile_chars:
-            groups += (acc + 15) // 16
-            acc = 0
-        acc += int(ln)
-    groups += (acc + 15) // 16
+    groups, chunks = packed_layout(ds, tile_chars)
     n_s, n_multi = len(lens), int((lens > 1).sum())
-    V = int(a["V"])
-    return 2 * groups * slots * 4 + 2 * (n_s + 2 * n_multi) * slots * score_bytes + V + V // 8 + 16 * n_s
+    return 2 * groups * slots * 4 + 2 * (2 * n_s + n_multi) * slots * score_bytes + 128 * chunks
 
 
 def profile_summary(kernel: str, bytes_per_launch: int) -> dict:
@@ -106,8 +130,10 @@ def profile_summary(kernel: str, bytes_per_launch: int) -> dict:
 
 
 # VALU issue rate of MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
-# instruction per SIMD every 4 cycles (16 lanes)
+# instruction per SIMD every 4 cycles (16 lanes); SALU: one instruction per
+# CU per cycle
 VALU_PEAK_INSTS = 256 * 4 * 2.4e9 / 4
+SALU_PEAK_INSTS = 256 * 2.4e9
 
 
 def row_bytes_survey(info: dict) -> int:
@@ -294,14 +320,18 @@ def main() -> int:
     pb = row_kernel_bytes(info, score_bytes)
     achieved = kb / (b_ms * 1e-3) / 1e9
     prof = profile_summary(kernel, kb)
-    valu = None
+    valu = salu = None
     if prof.get("valu_insts_per_launch"):
         at_peak = prof["valu_insts_per_launch"] / VALU_PEAK_INSTS
         valu = {"insts_per_launch": prof["valu_insts_per_launch"], "peak_insts_per_s": VALU_PEAK_INSTS,
                 "us_at_peak": round(at_peak * 1e6, 1), "frac": round(at_peak / (b_ms * 1e-3), 4)}
+    if prof.get("salu_insts_per_launch"):
+        at_peak = prof["salu_insts_per_launch"] / SALU_PEAK_INSTS
+        salu = {"insts_per_launch": prof["salu_insts_per_launch"], "peak_insts_per_s": SALU_PEAK_INSTS,
+                "us_at_peak": round(at_peak * 1e6, 1), "frac": round(at_peak / (b_ms * 1e-3), 4)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("traffic_bytes_per_launch"),
-                "valu": valu, "kernel": kernel, "kernel_avg_us": round(b_ms * 1e3, 2),
+                "valu": valu, "salu": salu, "kernel": kernel, "kernel_avg_us": round(b_ms * 1e3, 2),
                 "bytes_per_launch": kb, "launches_timed": stb["row_kernel_launches"],
                 "score_storage_bytes": score_bytes,
                 "plain_kernel": {"kernel": "k_row_tile", "avg_us": round(p_ms * 1e3, 2), "bytes_per_launch": pb,
