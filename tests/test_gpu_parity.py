@@ -113,12 +113,20 @@ def test_batch_schedules_are_exact(sched):
 
 
 def test_autotune_keeps_the_fastest_schedule():
+    """The tuner keeps the fastest candidate up to timing noise: in trial
+    order, a candidate displaces the kept schedule only when it is more than
+    1% faster, so the kept one is within 1/0.99 of the fastest."""
     ds, rs = small_workload(seed=38, n_reads=40, length=80, error=0.04)
     eng = Engine(ds)
     tried = eng.autotune(rs, ticks=8)
     assert len(tried) >= 10 and all(t["us_per_tick"] > 0 for t in tried)
-    best = min(tried, key=lambda t: t["us_per_tick"])
-    assert eng.schedule() == {k: best[k] for k in ("tile_chars", "big_level", "big_vpw")}
+    kept = tried[0]
+    for t in tried[1:]:
+        if t["us_per_tick"] < 0.99 * kept["us_per_tick"]:
+            kept = t
+    keys = ("tile_chars", "big_level", "big_vpw")
+    assert eng.schedule() == {k: kept[k] for k in keys}
+    assert kept["us_per_tick"] <= min(t["us_per_tick"] for t in tried) / 0.99
     assert np.array_equal(eng.score_batch(rs), O.consga_scores(ds.desc(), rs))
     with pytest.raises(N.NativeError):
         eng.set_schedule(16, 1, 4)
