@@ -39,8 +39,27 @@ class Dataset:
                                                    C.byref(h)))
         return cls(h.value)
 
+    @classmethod
+    def load_gfa(cls, path: str) -> "Dataset":
+        """GFA 1 graph, both orientations of every segment
+        (chsga_host_load_gfa)."""
+        h = C.c_void_p()
+        N._check_host(N.host_lib().chsga_host_load_gfa(str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_string_csr(cls, path: str) -> "Dataset":
+        """The string-labelled CSR text of handlegraph_to_csr
+        (chsga_host_load_string_csr)."""
+        h = C.c_void_p()
+        N._check_host(N.host_lib().chsga_host_load_string_csr(str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
     def save(self, prefix: str) -> None:
         N._check_host(N.host_lib().chsga_host_save(self._h, str(prefix).encode()))
+
+    def write_csr(self, string_csr: str, char_csr: str) -> None:
+        N._check_host(N.host_lib().chsga_host_write_csr(self._h, str(string_csr).encode(), str(char_csr).encode()))
 
     def rebuild_ch(self, seed: int) -> None:
         N._check_host(N.host_lib().chsga_host_build_ch(self._h, seed))

@@ -77,9 +77,12 @@ def build(verbose: bool = False) -> None:
         ("ConSGA_traceback_mismatchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=5"]),
         ("ConSGA_traceback_matchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=4"]),
         ("build_pch", "build_pch.cpp", []),
+        ("gfa_to_csr", "gfa_to_csr.cpp", []),
+        ("csr_to_adj", "csr_to_adj.cpp", []),
     ]:
         if (drv / src).exists():
-            libs = ["-lchsga_host"] if name == "build_pch" else ["-lchsga", "-lchsga_host"]
+            libs = {"build_pch": ["-lchsga_host"], "gfa_to_csr": ["-lchsga_host"], "csr_to_adj": []}.get(
+                name, ["-lchsga", "-lchsga_host"])
             stage2.append([CXX, *CXX_FLAGS, *defs, "-I", INC, drv / src, "-L", LIB, *libs, rpath, "-o", BIN / name])
     if stage2:
         _run(stage2, verbose)

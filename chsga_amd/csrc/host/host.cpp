@@ -2,6 +2,9 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <array>
+#include <functional>
+#include <unordered_map>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -589,6 +592,232 @@ int chsga_host_save(const chsga_dataset* ds, const char* prefix) {
       std::fprintf(f, "%llu\n", (unsigned long long)ds->str_off[ds->n_s]);
       std::fclose(f);
     }
+  })
+}
+
+// ---------------------------------------------------------------------------
+// converters: GFA and the string-labelled CSR text of handlegraph_to_csr
+// ---------------------------------------------------------------------------
+
+// Complement of one sequence byte as a handle graph reports a reverse
+// strand: IUPAC codes complemented, case kept, other bytes unchanged.
+static char complement(char c) {
+  switch (c) {
+    case 'A': return 'T';
+    case 'C': return 'G';
+    case 'G': return 'C';
+    case 'T': return 'A';
+    case 'U': return 'A';
+    case 'R': return 'Y';
+    case 'Y': return 'R';
+    case 'K': return 'M';
+    case 'M': return 'K';
+    case 'B': return 'V';
+    case 'V': return 'B';
+    case 'D': return 'H';
+    case 'H': return 'D';
+    case 'a': return 't';
+    case 'c': return 'g';
+    case 'g': return 'c';
+    case 't': return 'a';
+    case 'u': return 'a';
+    case 'r': return 'y';
+    case 'y': return 'r';
+    case 'k': return 'm';
+    case 'm': return 'k';
+    case 'b': return 'v';
+    case 'v': return 'b';
+    case 'd': return 'h';
+    case 'h': return 'd';
+    default: return c;  // N, S, W and others are their own complement
+  }
+}
+
+// Dataset from a string graph (sequences, out-lists): labels and offsets,
+// the char expansion, the CH.
+static void dataset_from_strings(const std::vector<std::string>& seq, std::vector<std::vector<uint32_t>>& out,
+                                 chsga_dataset& ds) {
+  const int64_t n = int64_t(seq.size());
+  if (n == 0) throw std::runtime_error("empty graph");
+  ds.n_s = n;
+  ds.str_off.assign(size_t(n) + 1, 0);
+  ds.labels.clear();
+  for (int64_t s = 0; s < n; ++s) {
+    if (seq[s].empty()) throw std::runtime_error("string vertex " + std::to_string(s) + " has an empty sequence");
+    ds.labels += seq[s];
+    ds.str_off[s + 1] = ds.labels.size();
+  }
+  ds.str_out_off.assign(size_t(n) + 1, 0);
+  ds.str_out_dst.clear();
+  for (int64_t s = 0; s < n; ++s) {
+    std::sort(out[s].begin(), out[s].end());
+    out[s].erase(std::unique(out[s].begin(), out[s].end()), out[s].end());
+    for (uint32_t t : out[s]) {
+      if (int64_t(t) >= n) throw std::runtime_error("edge to vertex " + std::to_string(t) + " out of range");
+      ds.str_out_dst.push_back(t);
+    }
+    ds.str_out_off[s + 1] = ds.str_out_dst.size();
+  }
+  expand_char_graph(ds);
+  build_ch(ds, 1);
+  validate_dataset(ds);
+}
+
+// GFA 1 (S and L lines, overlaps * or 0M), listed as handlegraph_to_csr
+// lists a handle graph (src/handlegraph_to_csr.cpp:39-110): segments ordered
+// by id (the numeric name, or file order when a name is not a number),
+// segment k's forward strand as string vertex 2k and its reverse complement
+// as 2k + 1; a link a -> b also gives flip(b) -> flip(a).
+static void load_gfa(const char* path, chsga_dataset& ds) {
+  const std::string text = slurp(path);
+  std::vector<std::string> name, seq;
+  std::vector<std::array<std::string, 4>> links;
+  std::istringstream in(text);
+  std::string line;
+  int64_t ln = 0;
+  while (std::getline(in, line)) {
+    ++ln;
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.empty() || (line[0] != 'S' && line[0] != 'L')) continue;
+    std::vector<std::string> f;
+    for (size_t a = 0;;) {
+      const size_t b = line.find('\t', a);
+      f.push_back(line.substr(a, b == std::string::npos ? std::string::npos : b - a));
+      if (b == std::string::npos) break;
+      a = b + 1;
+    }
+    auto bad = [&](const std::string& why) {
+      throw std::runtime_error(std::string(path) + ":" + std::to_string(ln) + ": " + why);
+    };
+    if (f[0] == "S") {
+      if (f.size() < 3) bad("S line needs a name and a sequence");
+      if (f[2] == "*" || f[2].empty()) bad("segment " + f[1] + " has no sequence");
+      name.push_back(f[1]);
+      seq.push_back(f[2]);
+    } else if (f[0] == "L") {
+      if (f.size() < 5) bad("L line needs from, orientation, to, orientation");
+      if ((f[2] != "+" && f[2] != "-") || (f[4] != "+" && f[4] != "-")) bad("orientation must be + or -");
+      if (f.size() > 5 && f[5] != "*" && f[5] != "0M") bad("only blunt links (overlap 0M or *) are supported");
+      links.push_back({f[1], f[2], f[3], f[4]});
+    }
+  }
+  const size_t k = name.size();
+  std::vector<size_t> order(k);
+  for (size_t i = 0; i < k; ++i) order[i] = i;
+  bool numeric = true;
+  for (const std::string& s : name)
+    numeric = numeric && !s.empty() && s.size() < 19 && std::all_of(s.begin(), s.end(), [](char c) {
+      return c >= '0' && c <= '9';
+    });
+  if (numeric)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return std::stoull(name[a]) < std::stoull(name[b]); });
+  std::unordered_map<std::string, uint32_t> idx;  // name -> position in id order
+  for (size_t i = 0; i < k; ++i)
+    if (!idx.emplace(name[order[i]], uint32_t(i)).second) throw std::runtime_error("duplicate segment " + name[order[i]]);
+  std::vector<std::string> sv(2 * k);
+  for (size_t i = 0; i < k; ++i) {
+    const std::string& s = seq[order[i]];
+    sv[2 * i] = s;
+    sv[2 * i + 1].assign(s.rbegin(), s.rend());
+    for (char& c : sv[2 * i + 1]) c = complement(c);
+  }
+  std::vector<std::vector<uint32_t>> out(2 * k);
+  for (const auto& l : links) {
+    const auto a = idx.find(l[0]), b = idx.find(l[2]);
+    if (a == idx.end() || b == idx.end()) throw std::runtime_error("link names an unknown segment");
+    const uint32_t h1 = 2 * a->second + (l[1] == "-" ? 1u : 0u);
+    const uint32_t h2 = 2 * b->second + (l[3] == "-" ? 1u : 0u);
+    out[h1].push_back(h2);
+    out[h2 ^ 1u].push_back(h1 ^ 1u);
+  }
+  dataset_from_strings(sv, out, ds);
+}
+
+// The string-labelled CSR text handlegraph_to_csr writes (write_csr,
+// src/handlegraph_to_csr.cpp:112-150): n, m, n labels, n + 1 offsets, m
+// destinations, whitespace separated.
+static void load_string_csr(const char* path, chsga_dataset& ds) {
+  const std::string text = slurp(path);
+  Tok t(text);
+  uint64_t n = 0, m = 0;
+  if (!t.next_u64(n) || !t.next_u64(m)) throw std::runtime_error(std::string(path) + ": expected n and m");
+  std::vector<std::string> seq(n);
+  for (uint64_t i = 0; i < n; ++i)
+    if (!t.next(seq[i])) throw std::runtime_error(std::string(path) + ": truncated labels");
+  std::vector<uint64_t> off(n + 1);
+  for (uint64_t i = 0; i <= n; ++i)
+    if (!t.next_u64(off[i])) throw std::runtime_error(std::string(path) + ": truncated offsets");
+  if (off[0] != 0 || off[n] != m) throw std::runtime_error(std::string(path) + ": offsets must run from 0 to m");
+  std::vector<std::vector<uint32_t>> out(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    if (off[i + 1] < off[i]) throw std::runtime_error(std::string(path) + ": decreasing offsets");
+    for (uint64_t e = off[i]; e < off[i + 1]; ++e) {
+      uint64_t d;
+      if (!t.next_u64(d)) throw std::runtime_error(std::string(path) + ": truncated adjacency");
+      out[i].push_back(uint32_t(d));
+    }
+  }
+  dataset_from_strings(seq, out, ds);
+}
+
+// A CSR text in that format: n, m, the labels, n + 1 offsets, m destinations.
+static void write_csr_text(const std::string& path, int64_t n, const std::function<void(std::FILE*, int64_t)>& label,
+                           const std::vector<uint64_t>& off, const std::vector<uint32_t>& dst) {
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot write " + path);
+  std::fprintf(f, "%lld\n%llu\n", (long long)n, (unsigned long long)dst.size());
+  for (int64_t v = 0; v < n; ++v) {
+    label(f, v);
+    std::fputc(' ', f);
+  }
+  std::fputc('\n', f);
+  for (int64_t v = 0; v <= n; ++v) std::fprintf(f, "%llu ", (unsigned long long)off[v]);
+  std::fputc('\n', f);
+  for (uint32_t d : dst) std::fprintf(f, "%u ", d);
+  std::fputc('\n', f);
+  if (std::fclose(f) != 0) throw std::runtime_error("cannot write " + path);
+}
+
+int chsga_host_load_gfa(const char* path, chsga_dataset** out) {
+  if (!path || !out) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  CHSGA_GUARD({
+    auto ds = std::make_unique<chsga_dataset>();
+    load_gfa(path, *ds);
+    *out = ds.release();
+  })
+}
+
+int chsga_host_load_string_csr(const char* path, chsga_dataset** out) {
+  if (!path || !out) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  CHSGA_GUARD({
+    auto ds = std::make_unique<chsga_dataset>();
+    load_string_csr(path, *ds);
+    *out = ds.release();
+  })
+}
+
+int chsga_host_write_csr(const chsga_dataset* ds, const char* string_csr, const char* char_csr) {
+  if (!ds || !string_csr || !char_csr) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  CHSGA_GUARD({
+    write_csr_text(
+        string_csr, ds->n_s,
+        [&](std::FILE* f, int64_t s) {
+          std::fwrite(ds->labels.data() + ds->str_off[s], 1, size_t(ds->str_off[s + 1] - ds->str_off[s]), f);
+        },
+        ds->str_out_off, ds->str_out_dst);
+    write_csr_text(
+        char_csr, ds->V, [&](std::FILE* f, int64_t p) { std::fputc(ds->labels[size_t(p)], f); }, ds->char_out_off,
+        ds->char_out_dst);
   })
 }
 

@@ -148,6 +148,9 @@ HOST_SYMBOLS = {
     "chsga_host_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                   C.POINTER(C.c_void_p)]),
     "chsga_host_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "chsga_host_load_gfa": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "chsga_host_load_string_csr": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "chsga_host_write_csr": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "chsga_host_save_ch": (C.c_int, [C.c_void_p, C.c_char_p]),
     "chsga_host_build_ch_file": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64]),
     "chsga_host_describe": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(GraphDesc)]),

@@ -12,6 +12,11 @@
  *   chsga_host_save        <- csr_to_adj (src/csr_to_adj.cpp:46-114) and
  *                             write_pbbs_format (build_pch.cpp:126)
  *   chsga_host_load_reads  <- load_reads (src/load_reads.hpp:7-40)
+ *   chsga_host_load_gfa,
+ *   chsga_host_load_string_csr,
+ *   chsga_host_write_csr   <- handlegraph_to_csr (src/handlegraph_to_csr.cpp:39-150
+ *                             and its char expansion, :165-247), reading GFA
+ *                             in place of a libbdsg PackedGraph
  *   chsga_host_generate,
  *   chsga_host_gen_reads   <- no counterpart (the reference ships no data)
  */
@@ -57,6 +62,21 @@ int chsga_host_load(const char* char_adj, const char* str_adj, const char* ch_pa
                     const char* offsets, chsga_dataset** out);
 /* Writes <prefix>.char.adj, .str.adj, .ch, .labels, .label_offsets. */
 int chsga_host_save(const chsga_dataset* ds, const char* prefix);
+/* GFA 1 input (S and L lines; link overlaps * or 0M): every segment in both
+ * orientations, as handlegraph_to_csr lists a handle graph
+ * (src/handlegraph_to_csr.cpp:39-110): segments ordered by id (the numeric
+ * name, or file order when a name is not a number), segment k's forward
+ * strand as string vertex 2k and its reverse complement as 2k + 1, a link
+ * a -> b also giving flip(b) -> flip(a), out-lists sorted; then the char
+ * expansion (handlegraph_to_csr.cpp:165-247) and the CH. */
+int chsga_host_load_gfa(const char* path, chsga_dataset** out);
+/* The string-labelled CSR text handlegraph_to_csr writes (write_csr,
+ * src/handlegraph_to_csr.cpp:112-150): n, m, n labels, n + 1 offsets, m
+ * destinations; then the char expansion and the CH. */
+int chsga_host_load_string_csr(const char* path, chsga_dataset** out);
+/* The string graph and its char expansion as two CSR texts in that format
+ * (handlegraph_to_csr's two outputs). */
+int chsga_host_write_csr(const chsga_dataset* ds, const char* string_csr, const char* char_csr);
 int chsga_host_save_ch(const chsga_dataset* ds, const char* path);
 /* build_pch: CH of a string graph file; label_offsets may be NULL, which
  * gives every string vertex length 1 (no vertex weights, as build_pch without
