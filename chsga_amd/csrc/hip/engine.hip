@@ -450,6 +450,74 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     };
     upload(c->d_bup_rec, records(up_off, bup, c->h_up_maxp, c->h_up_rec, c->h_up_rbase), s);
     upload(c->d_bdn_rec, records(dn_off, bdn, c->h_dn_maxp, c->h_dn_rec, c->h_dn_rbase), s);
+    // The top of the hierarchy in one launch.  Levels L0 .. J - 1 (at most
+    // kTopClosureMax vertices) pull only from each other and from lower
+    // levels, whose values are final when the up sweep reaches L0: their up
+    // then down sweep is a fixed min-plus map from source values to the
+    // final first values of the top vertices.  Running the pulls on symbols
+    // (one-char top vertices relay what reached them; a multi-char source
+    // gives its stale last value, a lower vertex its pull value) gives each
+    // top vertex a list of {source, summed weight} terms, and one level
+    // launch over those lists replaces the top levels' launches.  Weights
+    // compose linearly (each pull adds w * ins), so the result is the
+    // level-by-level sweep's.  Terms whose weight * ins reaches 2^30 are
+    // dropped: no row value reaches them.
+    c->top_l0 = c->J;
+    {
+      const int32_t J = c->J;
+      int32_t L0 = J;
+      while (L0 > 1 && c->h_lvl_off[J] - c->h_lvl_off[L0 - 1] <= uint32_t(kTopClosureMax)) --L0;
+      if (L0 <= J - 2) {
+        const uint32_t kT0 = c->h_lvl_off[L0], kT1 = c->h_lvl_off[J];
+        using Terms = std::vector<std::pair<uint32_t, int64_t>>;
+        std::vector<Terms> terms(kT1 - kT0);
+        auto add = [](Terms& t, uint32_t src, int64_t o) {
+          for (auto& e : t)
+            if (e.first == src) {
+              e.second = std::min(e.second, o);
+              return;
+            }
+          t.emplace_back(src, o);
+        };
+        for (uint32_t k = kT0; k < kT1; ++k) terms[k - kT0].emplace_back(vfirst[k], 0);
+        auto pull = [&](uint32_t k, const std::vector<uint32_t>& off, const std::vector<int2>& pin) {
+          for (uint32_t e = off[k]; e < off[k + 1]; ++e) {
+            const uint32_t x = uint32_t(pin[e].x) & 0x7fffffffu;
+            const int64_t w = pin[e].y;
+            const uint32_t px = pos_of_og[x];
+            if (c->h_str_off[x + 1] - c->h_str_off[x] == 1 && px >= kT0) {
+              const Terms src = terms[px - kT0];  // a one-char top vertex relays its terms
+              for (const auto& t : src) add(terms[k - kT0], t.first, t.second + w);
+            } else {
+              add(terms[k - kT0], uint32_t(pull_src(x)), w);
+            }
+          }
+        };
+        for (int32_t j = L0; j < J; ++j)
+          for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, up_off, up_in);
+        for (int32_t j = J - 2; j >= L0; --j)
+          for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, dn_off, dn_in);
+        std::vector<uint32_t> toff(size_t(n_s) + 1 + kSweepPad, 0u);
+        std::vector<int2> tin;
+        uint32_t tmax = 0;
+        for (uint32_t k = 0; k <= uint32_t(n_s); ++k) {
+          toff[k] = uint32_t(tin.size());
+          if (k < kT0 || k >= kT1) continue;
+          for (const auto& t : terms[k - kT0]) {
+            if (t.first == vfirst[k] && t.second == 0) continue;  // the vertex's own value
+            if (int64_t(c->isc) * t.second >= (int64_t(1) << 30)) continue;
+            tin.push_back(make_int2(int32_t(t.first), int32_t(t.second)));
+          }
+          tmax = std::max(tmax, uint32_t(tin.size()) - toff[k]);
+        }
+        for (size_t k = size_t(n_s) + 1; k < toff.size(); ++k) toff[k] = uint32_t(tin.size());
+        tin.resize(tin.size() + tmax + 2 * kSweepChunk, make_int2(0, 0));
+        upload(c->d_top_off, toff, s);
+        upload(c->d_top_in, tin, s);
+        c->top_l0 = L0;
+        c->top_maxp = tmax;
+      }
+    }
   }
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);
@@ -720,6 +788,9 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         break;
       case CHSGA_OPT_SEG_ROWS:
         ctx->seg_rows_cap = value < 0 ? -1 : value;
+        break;
+      case CHSGA_OPT_TOP_CLOSURE:
+        ctx->top_closure = value != 0;
         break;
       case CHSGA_OPT_PACKED_ROWS:
         if ((value != 0) != ctx->packed)

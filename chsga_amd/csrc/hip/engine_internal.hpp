@@ -58,6 +58,8 @@ constexpr int kSweepChunk = 16;
 // words of a batched sweep pull record with R pull slots: first char, R
 // sources, R 16-bit weights, rounded up to a power of two
 constexpr int sweep_record_words(int R) { return R == 2 ? 4 : R == 4 ? 8 : 16; }
+// most vertices of the top CH levels the batched sweep runs as one closure
+constexpr int64_t kTopClosureMax = 16384;
 // chars the batched row buffers hold past V (>= one row-pass chunk)
 constexpr int64_t kRowPad = 32;
 struct TopSet {
@@ -114,6 +116,12 @@ struct chsga_ctx {
   // with h_*_rec[j] = R in {2, 4, 8} (0: the level takes the pull lists),
   // level j's at word h_*_rbase[j]
   chsga_engine::DevMem d_bup_rec, d_bdn_rec;
+  // batched sweep: levels top_l0 .. J - 1 as one launch over precomputed
+  // {source, summed weight} lists by sweep position (top_l0 == J: none)
+  int32_t top_l0 = 0;
+  bool top_closure = true;  // CHSGA_OPT_TOP_CLOSURE
+  uint32_t top_maxp = 0;
+  chsga_engine::DevMem d_top_off, d_top_in;
   std::vector<uint8_t> h_up_rec, h_dn_rec;
   std::vector<uint64_t> h_up_rbase, h_dn_rbase;
   TopSet top_plain;

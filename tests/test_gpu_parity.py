@@ -314,6 +314,37 @@ def test_int16_storage_equals_int32_storage():
     eng.close()
 
 
+def test_top_levels_as_one_closure_equal_level_by_level():
+    """The batched sweep's top CH levels run as one launch over precomputed
+    {source, summed weight} lists (the default); scores and traceback paths
+    equal the level-by-level sweep's and the oracle's: on c1 (every level
+    above 0 in the closure), on small graphs of both alphabets, packed and
+    char rows, and with other insertion costs (the weights' multiplier)."""
+    w = WORKLOADS["c1"]
+    ds2, rs2 = small_workload(seed=53, n_reads=80, length=150, error=0.08, n_rate=0.03)
+    cases = [(w.dataset(), w.alphabet, None, None), (ds2, N.ALPHABET_MISMATCH_N, rs2, None),
+             (ds2, N.ALPHABET_MATCH_N, rs2, None)]
+    for ins in (0, 2):
+        d = ds2.desc()
+        d.ins_score = ins
+        cases.append((ds2, N.ALPHABET_MISMATCH_N, rs2, d))
+    for ds, alpha, rs, desc in cases:
+        if rs is None:
+            rs = w.readset(ds, n_reads=40)
+        rs = ReadSet.from_list([rs[i][: 1 + (i * 31) % len(rs[i])] for i in range(len(rs))])
+        for packed in (True, False):
+            a = Engine(ds, alpha, desc=desc, packed=packed)
+            _say("top closure vs level by level", alpha, packed, None if desc is None else desc.ins_score)
+            ref = O.consga_scores(desc if desc is not None else ds.desc(alpha), rs)
+            sa, ta = a.score_batch(rs, use_graphs=True), a.traceback_batch(rs, k=40)
+            a.set_option(N.OPT_TOP_CLOSURE, 0)
+            sb, tb = a.score_batch(rs, use_graphs=True), a.traceback_batch(rs, k=40)
+            assert np.array_equal(sa, ref) and np.array_equal(sb, ref)
+            for i in range(len(rs)):
+                assert ta[i].score == tb[i].score and np.array_equal(ta[i].ops, tb[i].ops), i
+            a.close()
+
+
 def test_packed_rows_equal_char_rows():
     """Batched rows packed as 2-bit deltas (the default on compact graphs
     with unit costs) give the scores and paths of one score per char, on c1,
