@@ -461,7 +461,10 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     // launch over those lists replaces the top levels' launches.  Weights
     // compose linearly (each pull adds w * ins), so the result is the
     // level-by-level sweep's.  Terms whose weight * ins reaches 2^30 are
-    // dropped: no row value reaches them.
+    // dropped: no row value reaches them.  Terms are keyed by the pull
+    // lists' string codes (string id, | 2^31 for the stale last value of a
+    // multi-char string), which the plain sweep reads from F / Lst; the
+    // batched sweep gets the matching row elements.
     c->top_l0 = c->J;
     {
       const int32_t J = c->J;
@@ -479,7 +482,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
             }
           t.emplace_back(src, o);
         };
-        for (uint32_t k = kT0; k < kT1; ++k) terms[k - kT0].emplace_back(vfirst[k], 0);
+        for (uint32_t k = kT0; k < kT1; ++k) terms[k - kT0].emplace_back(sw_vert[k], 0);
         auto pull = [&](uint32_t k, const std::vector<uint32_t>& off, const std::vector<int2>& pin) {
           for (uint32_t e = off[k]; e < off[k + 1]; ++e) {
             const uint32_t x = uint32_t(pin[e].x) & 0x7fffffffu;
@@ -489,24 +492,32 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
               const Terms src = terms[px - kT0];  // a one-char top vertex relays its terms
               for (const auto& t : src) add(terms[k - kT0], t.first, t.second + w);
             } else {
-              add(terms[k - kT0], uint32_t(pull_src(x)), w);
+              add(terms[k - kT0], uint32_t(pin[e].x), w);
             }
           }
+        };
+        // row element of a term: the first value of a string, or the stale
+        // last value of a multi-char one
+        auto element = [&](uint32_t code) -> int32_t {
+          const uint32_t x = code & 0x7fffffffu;
+          if (code >> 31) return pull_src(x);
+          return int32_t(c->packed ? 3 * x : c->h_str_off[x]);
         };
         for (int32_t j = L0; j < J; ++j)
           for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, up_off, up_in);
         for (int32_t j = J - 2; j >= L0; --j)
           for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, dn_off, dn_in);
         std::vector<uint32_t> toff(size_t(n_s) + 1 + kSweepPad, 0u);
-        std::vector<int2> tin;
+        std::vector<int2> tin, tin_plain;
         uint32_t tmax = 0;
         for (uint32_t k = 0; k <= uint32_t(n_s); ++k) {
           toff[k] = uint32_t(tin.size());
           if (k < kT0 || k >= kT1) continue;
           for (const auto& t : terms[k - kT0]) {
-            if (t.first == vfirst[k] && t.second == 0) continue;  // the vertex's own value
+            if (t.first == sw_vert[k] && t.second == 0) continue;  // the vertex's own value
             if (int64_t(c->isc) * t.second >= (int64_t(1) << 30)) continue;
-            tin.push_back(make_int2(int32_t(t.first), int32_t(t.second)));
+            tin.push_back(make_int2(element(t.first), int32_t(t.second)));
+            tin_plain.push_back(make_int2(int32_t(t.first), int32_t(t.second)));
           }
           tmax = std::max(tmax, uint32_t(tin.size()) - toff[k]);
         }
@@ -514,6 +525,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
         tin.resize(tin.size() + tmax + 2 * kSweepChunk, make_int2(0, 0));
         upload(c->d_top_off, toff, s);
         upload(c->d_top_in, tin, s);
+        upload(c->d_top_in_plain, tin_plain, s);
         c->top_l0 = L0;
         c->top_maxp = tmax;
       }
@@ -544,6 +556,24 @@ static void launch_sweeps_t(chsga_ctx* c, int mode) {
   const hipStream_t s = c->stream;
   T* F = (T*)c->d_F.p;
   const T* L = (const T*)c->d_Lst.p;
+  auto level = [&](uint32_t k0, uint32_t k1, const void* off, const void* in) {
+    hipLaunchKernelGGL(k_sweep_level<T>, dim3((k1 - k0 + CHSGA_BLOCK - 1) / CHSGA_BLOCK), dim3(CHSGA_BLOCK), 0, s, F, L,
+                       (const uint32_t*)c->d_sw_vert.p, (const uint32_t*)off, (const int2*)in, k0, k1, c->isc);
+    c->launches += 1;
+  };
+  if (mode == 3 && c->top_closure && c->top_l0 < c->J) {
+    // levels below top_l0 one by one, the top levels as one closure launch
+    for (int32_t j = 1; j < c->top_l0; ++j) {
+      const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
+      if (c->h_up_off[k1] != c->h_up_off[k0]) level(k0, k1, c->d_up_off.p, c->d_up_in.p);
+    }
+    level(c->h_lvl_off[c->top_l0], c->h_lvl_off[c->J], c->d_top_off.p, c->d_top_in_plain.p);
+    for (int32_t j = c->top_l0 - 1; j >= 0; --j) {
+      const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
+      if (c->h_dn_off[k1] != c->h_dn_off[k0]) level(k0, k1, c->d_dn_off.p, c->d_dn_in.p);
+    }
+    return;
+  }
   if (mode & 1)
     for (int32_t j = 1; j < c->j_top; ++j) {
       const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];

@@ -122,6 +122,7 @@ struct chsga_ctx {
   bool top_closure = true;  // CHSGA_OPT_TOP_CLOSURE
   uint32_t top_maxp = 0;
   chsga_engine::DevMem d_top_off, d_top_in;
+  chsga_engine::DevMem d_top_in_plain;  // the same lists as string codes (plain sweep: F / Lst)
   std::vector<uint8_t> h_up_rec, h_dn_rec;
   std::vector<uint64_t> h_up_rbase, h_dn_rbase;
   TopSet top_plain;

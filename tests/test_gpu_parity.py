@@ -316,8 +316,9 @@ def test_int16_storage_equals_int32_storage():
 
 def test_top_levels_as_one_closure_equal_level_by_level():
     """The batched sweep's top CH levels run as one launch over precomputed
-    {source, summed weight} lists (the default); scores and traceback paths
-    equal the level-by-level sweep's and the oracle's: on c1 (every level
+    {source, summed weight} lists (the default, also in the plain path's
+    full sweeps); scores and traceback paths equal the level-by-level
+    sweep's and the oracle's: on c1 (every level
     above 0 in the closure), on small graphs of both alphabets, packed and
     char rows, and with other insertion costs (the weights' multiplier)."""
     w = WORKLOADS["c1"]
@@ -337,9 +338,12 @@ def test_top_levels_as_one_closure_equal_level_by_level():
             _say("top closure vs level by level", alpha, packed, None if desc is None else desc.ins_score)
             ref = O.consga_scores(desc if desc is not None else ds.desc(alpha), rs)
             sa, ta = a.score_batch(rs, use_graphs=True), a.traceback_batch(rs, k=40)
+            pa = a.align_combo(rs)  # the plain path's full sweeps take the same lists
             a.set_option(N.OPT_TOP_CLOSURE, 0)
             sb, tb = a.score_batch(rs, use_graphs=True), a.traceback_batch(rs, k=40)
+            pb = a.align_combo(rs)
             assert np.array_equal(sa, ref) and np.array_equal(sb, ref)
+            assert np.array_equal(pa, ref) and np.array_equal(pb, ref)
             for i in range(len(rs)):
                 assert ta[i].score == tb[i].score and np.array_equal(ta[i].ops, tb[i].ops), i
             a.close()
