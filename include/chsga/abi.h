@@ -138,9 +138,9 @@ typedef struct chsga_stats {
                                      two levels (-1, default: whole when it fits in free memory) */
 #define CHSGA_OPT_PACKED_ROWS 10  /* read-only check: 1 if the batched rows are packed (setting a different
                                      value is an error: the choice is made by chsga_ctx_create_ex) */
-#define CHSGA_OPT_TOP_CLOSURE 11  /* 1 (default): the batched sweep runs the top CH levels (at most 16384
-                                     vertices) as one launch over precomputed closure lists; 0: level by
-                                     level (same results) */
+#define CHSGA_OPT_TOP_CLOSURE 11  /* 1 (default): the batched sweep and the plain path's full sweeps run the
+                                     top CH levels (at most 16384 vertices) as one launch over precomputed
+                                     closure lists; 0: level by level (same results) */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);
