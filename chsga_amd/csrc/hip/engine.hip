@@ -8,6 +8,7 @@
 #include <exception>
 #include <mutex>
 #include <numeric>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -251,6 +252,32 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     }
     dn_off[k + 1] = uint32_t(dn_in.size());
   }
+  // exact insertion closure (chsga_ctx_create_ex, CHSGA_CREATE_EXACT_CLOSURE:
+  // the semantics of ParSGA's serial DAG form, a topological insertion pass,
+  // src/ParSGAoverrides/csr_align.hpp:316-327): a multi-char source also
+  // gives its live last value, its first value plus len - 1 insertions, so
+  // every pull from one gets a twin on its first value at weight w + len - 1
+  if (c->exact) {
+    auto twin = [&](std::vector<uint32_t>& off, std::vector<int2>& in) {
+      std::vector<uint32_t> o2(off.size(), 0u);
+      std::vector<int2> i2;
+      i2.reserve(in.size() * 2);
+      for (int64_t k = 0; k < n_s; ++k) {
+        for (uint32_t e = off[k]; e < off[k + 1]; ++e) {
+          i2.push_back(in[e]);
+          if (in[e].x < 0) {
+            const uint32_t x = uint32_t(in[e].x) & 0x7fffffffu;
+            i2.push_back(make_int2(int32_t(x), in[e].y + int32_t(c->h_str_off[x + 1] - c->h_str_off[x] - 1)));
+          }
+        }
+        o2[k + 1] = uint32_t(i2.size());
+      }
+      off.swap(o2);
+      in.swap(i2);
+    };
+    twin(up_off, up_in);
+    twin(dn_off, dn_in);
+  }
   // first step of the plain path's LDS-resident top kernel: as many top
   // levels as fit kTopPlain string vertices
   auto top_step = [&](int64_t max_nt) {
@@ -474,27 +501,38 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
         const uint32_t kT0 = c->h_lvl_off[L0], kT1 = c->h_lvl_off[J];
         using Terms = std::vector<std::pair<uint32_t, int64_t>>;
         std::vector<Terms> terms(kT1 - kT0);
-        auto add = [](Terms& t, uint32_t src, int64_t o) {
-          for (auto& e : t)
-            if (e.first == src) {
-              e.second = std::min(e.second, o);
-              return;
-            }
-          t.emplace_back(src, o);
+        std::unordered_map<uint32_t, int64_t> acc;  // the terms of the vertex being pulled
+        auto add = [&](uint32_t src, int64_t o) {
+          const auto it = acc.find(src);
+          if (it == acc.end())
+            acc.emplace(src, o);
+          else
+            it->second = std::min(it->second, o);
         };
+        int64_t n_terms = 0;
+        bool fits = true;
         for (uint32_t k = kT0; k < kT1; ++k) terms[k - kT0].emplace_back(sw_vert[k], 0);
         auto pull = [&](uint32_t k, const std::vector<uint32_t>& off, const std::vector<int2>& pin) {
+          if (!fits) return;
+          Terms& tk = terms[k - kT0];
+          acc.clear();
+          for (const auto& t : tk) acc.emplace(t.first, t.second);
           for (uint32_t e = off[k]; e < off[k + 1]; ++e) {
             const uint32_t x = uint32_t(pin[e].x) & 0x7fffffffu;
             const int64_t w = pin[e].y;
             const uint32_t px = pos_of_og[x];
-            if (c->h_str_off[x + 1] - c->h_str_off[x] == 1 && px >= kT0) {
-              const Terms src = terms[px - kT0];  // a one-char top vertex relays its terms
-              for (const auto& t : src) add(terms[k - kT0], t.first, t.second + w);
+            if (pin[e].x >= 0 && px >= kT0) {
+              // the first value of a top vertex relays its terms (every
+              // pull of a one-char string; a multi-char string's twin entry
+              // with exact closure)
+              for (const auto& t : terms[px - kT0]) add(t.first, t.second + w);
             } else {
-              add(terms[k - kT0], uint32_t(pin[e].x), w);
+              add(uint32_t(pin[e].x), w);
             }
           }
+          n_terms += int64_t(acc.size()) - int64_t(tk.size());
+          tk.assign(acc.begin(), acc.end());
+          fits = n_terms <= kTopClosureTermsMax;
         };
         // row element of a term: the first value of a string, or the stale
         // last value of a multi-char one
@@ -507,6 +545,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
           for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, up_off, up_in);
         for (int32_t j = J - 2; j >= L0; --j)
           for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, dn_off, dn_in);
+        if (fits) {
         std::vector<uint32_t> toff(size_t(n_s) + 1 + kSweepPad, 0u);
         std::vector<int2> tin, tin_plain;
         uint32_t tmax = 0;
@@ -528,6 +567,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
         upload(c->d_top_in_plain, tin_plain, s);
         c->top_l0 = L0;
         c->top_maxp = tmax;
+        }
       }
     }
   }
@@ -766,11 +806,13 @@ int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, c
   *out = nullptr;
   chsga_ctx* c = nullptr;
   ENGINE_GUARD({
-    if (flags & ~uint32_t(CHSGA_CREATE_CHAR_ROWS)) throw std::invalid_argument("unknown context flags");
+    if (flags & ~uint32_t(CHSGA_CREATE_CHAR_ROWS | CHSGA_CREATE_EXACT_CLOSURE))
+      throw std::invalid_argument("unknown context flags");
     c = new chsga_ctx();
     try {
       c->device = device;
       c->allow_packed = (flags & CHSGA_CREATE_CHAR_ROWS) == 0;
+      c->exact = (flags & CHSGA_CREATE_EXACT_CLOSURE) != 0;
       build_context(c, g);
     } catch (...) {
       delete c;
@@ -808,6 +850,8 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         ctx->allow_narrow = value != 0;
         break;
       case CHSGA_OPT_STAGED_WALK:
+        if (value != 0 && ctx->exact)
+          throw std::invalid_argument("the staged walk follows the stale-last sweep; exact-closure rows need none");
         ctx->force_staged = value != 0;
         break;
       case CHSGA_OPT_BATCHED_BACKWARD:

@@ -60,6 +60,9 @@ constexpr int kSweepChunk = 16;
 constexpr int sweep_record_words(int R) { return R == 2 ? 4 : R == 4 ? 8 : 16; }
 // most vertices of the top CH levels the batched sweep runs as one closure
 constexpr int64_t kTopClosureMax = 16384;
+// most {source, weight} terms of that closure; more (long relay chains, as
+// exact closure can give) and the top levels keep their level launches
+constexpr int64_t kTopClosureTermsMax = int64_t(1) << 22;
 // chars the batched row buffers hold past V (>= one row-pass chunk)
 constexpr int64_t kRowPad = 32;
 struct TopSet {
@@ -100,6 +103,7 @@ struct chsga_ctx {
   bool nb2_all = false;  // compact, and no first char has more than 2 in-neighbours
   bool batch_fast = false;  // nb2_all and every label in ACGTN: the batched compact row kernel
   bool allow_packed = true;  // chsga_ctx_create_ex without CHSGA_CREATE_CHAR_ROWS
+  bool exact = false;        // CHSGA_CREATE_EXACT_CLOSURE: live last values in the sweep
   bool packed = false;       // batched rows packed as 2-bit deltas (batch_fast, unit costs)
   std::vector<int4> h_pbnb;        // packed: per string {string, length - 1} of its two in-neighbour strings
   std::vector<uint8_t> h_lclass;   // packed: match class of each char's label

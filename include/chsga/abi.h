@@ -157,6 +157,14 @@ int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out);
  * costs on compact graphs, the default there) apply.  Results are the same
  * either way. */
 #define CHSGA_CREATE_CHAR_ROWS 1u
+/* CHSGA_CREATE_EXACT_CLOSURE: the exact insertion closure of every row, the
+ * semantics of ParSGA's serial DAG form (a topological insertion pass,
+ * src/ParSGAoverrides/csr_align.hpp:316-327, row 0 without closure): the CH
+ * sweep takes a multi-char source's live last value (its first value plus
+ * len - 1 insertions) besides its stale one, so scores can be lower than
+ * ConSGA's.  Scores are exact int32 (the reference's ParSGA drivers keep
+ * uint8 scores, which wrap above 255). */
+#define CHSGA_CREATE_EXACT_CLOSURE 2u
 int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, chsga_ctx** out);
 void chsga_ctx_destroy(chsga_ctx* ctx);
 int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value);

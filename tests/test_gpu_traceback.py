@@ -267,3 +267,52 @@ def test_c5_traceback_validates():
     assert eng.last_stats["host_bytes"] > 0
     assert tb.score == out[0].score and np.array_equal(tb.ops, out[0].ops)
     eng.close()
+
+
+def test_exact_closure_matches_the_exact_oracle(tmp_path):
+    """CHSGA_CREATE_EXACT_CLOSURE, ParSGA's semantics (a topological
+    insertion pass per row, src/ParSGAoverrides/csr_align.hpp:316-327):
+    batched and plain scores equal the exact oracle (O1) on random string
+    DAGs of long and one-char strings, a generated graph and c1; traceback
+    paths validate at those scores; whole last rows (plain path) equal the
+    oracle's exact rows and differ from ConSGA's on some read."""
+    rng = np.random.default_rng(21)
+    cases = []
+    for t in range(3):
+        strings, edges = R.random_string_dag(rng, 40, 6 if t else 1, p_edge=0.1)
+        p = R.write_dataset(tmp_path, strings, edges, f"ex{t}")
+        ds = Dataset.load(p["char.adj"], p["str.adj"], None, p["labels"], p["label_offsets"])
+        cases.append((ds, ReadSet.from_list([R.rbytes(rng, int(rng.integers(1, 40))) for _ in range(70)])))
+    cases.append(small_workload(seed=61, n_reads=40, length=120, error=0.1, n_rate=0.02))
+    w = WORKLOADS["c1"]
+    ds1 = w.dataset()
+    r1 = w.readset(ds1, n_reads=12)
+    cases.append((ds1, ReadSet.from_list([r1[i][:300] for i in range(len(r1))])))
+    rows_differ = 0
+    for ds, rs in cases:
+        d = ds.desc()
+        ref = O.exact_scores(d, rs)
+        for packed in (True, False):
+            eng = Engine(ds, exact=True, packed=packed)
+            print("exact closure", ds.info()["n_s"], packed, flush=True)
+            assert np.array_equal(eng.score_batch(rs, use_graphs=True), ref)
+            assert np.array_equal(eng.align_combo(rs), ref)
+            eng.set_option(N.OPT_TOP_CLOSURE, 0)
+            assert np.array_equal(eng.score_batch(rs), ref)
+            eng.set_option(N.OPT_TOP_CLOSURE, 1)
+            out = eng.traceback_batch(rs, k=16)
+            assert eng.last_stats["walk_fallbacks"] == 0
+            for i in range(len(rs)):
+                assert out[i].score == ref[i]
+                assert O.validate_path(d, rs[i], out[i].ops, out[i].score)[0] == 0, i
+            tb = eng.align_read_with_traceback(rs[0], k=5)
+            assert tb.score == ref[0] and O.validate_path(d, rs[0], tb.ops, tb.score)[0] == 0
+            with pytest.raises(N.NativeError):
+                eng.set_option(N.OPT_STAGED_WALK, 1)
+            if packed:
+                for i in range(min(len(rs), 8)):
+                    row = eng.last_row(rs[i])
+                    assert np.array_equal(row, O.exact_last_row(d, rs[i])), i
+                    rows_differ += int((row != O.consga_last_row(d, rs[i])).any())
+            eng.close()
+    assert rows_differ > 0
