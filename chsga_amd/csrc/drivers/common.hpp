@@ -117,8 +117,10 @@ inline void load_inputs(const char* I, const char* i, const char* c, const char*
   chsga_host_describe(in.ds, CHSGA_ALPHABET, &in.desc);
 }
 
-inline void open_engine(Inputs& in, int device) {
-  if (chsga_ctx_create(&in.desc, device, &in.ctx) != CHSGA_OK) die("Error: ", chsga_last_error());
+// exact: ParSGA's semantics (CHSGA_CREATE_EXACT_CLOSURE) instead of ConSGA's
+inline void open_engine(Inputs& in, int device, bool exact = false) {
+  if (chsga_ctx_create_ex(&in.desc, device, exact ? CHSGA_CREATE_EXACT_CLOSURE : 0u, &in.ctx) != CHSGA_OK)
+    die("Error: ", chsga_last_error());
 }
 
 }  // namespace drv
