@@ -289,6 +289,16 @@ def test_drop_in_drivers(tmp_path):
     out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-b"], capture_output=True, text=True,
                          timeout=300)
     assert out.returncode == 0 and " Substitutions: NA sec" in out.stdout
+    # -E: ParSGA's exact insertion closure
+    exact = O.exact_scores(ds.desc(), rs)
+    out = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-E", "-B"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.splitlines()[-1] == "score: " + " ".join(str(int(x)) for x in exact)
+    out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-k", "100", "-B", "-E"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    assert "[RESULT] ALL CHECKS PASSED" in out.stdout
     bad = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-a", "x"], capture_output=True, text=True)
     assert bad.returncode != 0 and "Usage" in bad.stderr
 
