@@ -300,6 +300,21 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_in_src, c->h_in_src, s);
   upload(c->d_first_bits, first_bits, s);
   if (c->compact) {
+    // plain compact kernel: at each first char its first two in-neighbours
+    // (0xffffffff: none), bit 31 of .x set when it has more (read from the
+    // in-CSR), so its gathers wait for one load instead of two
+    std::vector<int2> nb(size_t(V) + 8, make_int2(-1, -1));
+    for (int64_t st = 0; st < n_s; ++st) {
+      const uint32_t p = c->h_str_off[st];
+      const uint32_t e0 = c->h_in_off[p], e1 = c->h_in_off[p + 1];
+      uint32_t x = e1 > e0 ? c->h_in_src[e0] : 0xffffffffu;
+      const uint32_t y = e1 > e0 + 1 ? c->h_in_src[e0 + 1] : 0xffffffffu;
+      if (e1 > e0 + 2) x |= 0x80000000u;
+      nb[p] = make_int2(int32_t(x), int32_t(y));
+    }
+    upload(c->d_first_nb, nb, s);
+  }
+  if (c->compact) {
     // batched compact kernel: for every string, the last char (and, for a
     // string of more than one char, the first char) of the at most two
     // strings whose last char feeds its first char
@@ -587,8 +602,9 @@ static void launch_row_t(chsga_ctx* c, const void* H, void* D, uint8_t rc) {
   hipLaunchKernelGGL((c->compact ? k_row_tile8<true, T> : k_row_tile8<false, T>), dim3(unsigned(c->n_tiles)),
                      dim3(CHSGA_BLOCK), 0, s, (const T*)H, (T*)D, (T*)c->d_F.p, (T*)c->d_Lst.p,
                      (const uint8_t*)c->d_labels.p, (const uint32_t*)c->d_in_off.p, (const uint32_t*)c->d_in_src.p,
-                     (const uint32_t*)c->d_first_bits.p, (const uint32_t*)c->d_tile_char.p,
-                     (const uint32_t*)c->d_tile_str.p, c->V, rc, c->alphabet, c->msc, c->ssc, c->dsc, c->isc);
+                     (const int2*)c->d_first_nb.p, (const uint32_t*)c->d_first_bits.p,
+                     (const uint32_t*)c->d_tile_char.p, (const uint32_t*)c->d_tile_str.p, c->V, rc, c->alphabet,
+                     c->msc, c->ssc, c->dsc, c->isc);
 }
 
 template <typename T>

@@ -109,6 +109,7 @@ struct chsga_ctx {
   std::vector<uint8_t> h_lclass;   // packed: match class of each char's label
   chsga_engine::DevMem d_cstr;  // packed: string of each char (batched traceback walk)
   chsga_engine::DevMem d_lclass;  // match class of each char's label (batched compact kernel)
+  chsga_engine::DevMem d_first_nb;  // plain compact kernel: first two in-neighbours of each first char
   chsga_engine::DevMem d_bnb;     // per string: {last, first} chars of <= 2 in-neighbour strings
   chsga_engine::DevMem d_cfirst;  // first char of each char's string (general batched kernel)
   chsga_engine::DevMem d_sw_vert, d_lvl_off, d_up_off, d_up_in, d_dn_off, d_dn_in;
