@@ -882,6 +882,9 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
       case CHSGA_OPT_TOP_CLOSURE:
         ctx->top_closure = value != 0;
         break;
+      case CHSGA_OPT_RING_ROWS:
+        ctx->ring_rows_cap = value < 0 ? -1 : value;
+        break;
       case CHSGA_OPT_PACKED_ROWS:
         if ((value != 0) != ctx->packed)
           throw std::invalid_argument(ctx->packed ? "the batched rows are packed"

@@ -141,6 +141,10 @@ typedef struct chsga_stats {
 #define CHSGA_OPT_TOP_CLOSURE 11  /* 1 (default): the batched sweep and the plain path's full sweeps run the
                                      top CH levels (at most 16384 vertices) as one launch over precomputed
                                      closure lists; 0: level by level (same results) */
+#define CHSGA_OPT_RING_ROWS 12    /* chsga_traceback_batch, batched backward: at most this many rows in the
+                                     ring that pass B recomputes into (-1, default: the plan with the fewest
+                                     recomputed rows whose buffers fit in free memory; never fewer than
+                                     ceil(sqrt(k)) rows) */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);
