@@ -507,12 +507,17 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
     // lists' string codes (string id, | 2^31 for the stale last value of a
     // multi-char string), which the plain sweep reads from F / Lst; the
     // batched sweep gets the matching row elements.
-    c->top_l0 = c->J;
-    {
+    // The batched sweep (64 slots a vertex) and the plain sweep (one read)
+    // get closures of their own size: a longer closure list costs the
+    // batched launch more than the level launches it replaces, the plain
+    // path's level launches cost their launch latency.
+    c->top_l0 = c->top_l0_plain = c->J;
+    auto build_top = [&](int64_t cap, bool plain) {
       const int32_t J = c->J;
       int32_t L0 = J;
-      while (L0 > 1 && c->h_lvl_off[J] - c->h_lvl_off[L0 - 1] <= uint32_t(kTopClosureMax)) --L0;
-      if (L0 <= J - 2) {
+      while (L0 > 1 && c->h_lvl_off[J] - c->h_lvl_off[L0 - 1] <= uint32_t(cap)) --L0;
+      {
+        if (L0 > J - 2) return;
         const uint32_t kT0 = c->h_lvl_off[L0], kT1 = c->h_lvl_off[J];
         using Terms = std::vector<std::pair<uint32_t, int64_t>>;
         std::vector<Terms> terms(kT1 - kT0);
@@ -562,7 +567,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
           for (uint32_t k = c->h_lvl_off[j]; k < c->h_lvl_off[j + 1]; ++k) pull(k, dn_off, dn_in);
         if (fits) {
         std::vector<uint32_t> toff(size_t(n_s) + 1 + kSweepPad, 0u);
-        std::vector<int2> tin, tin_plain;
+        std::vector<int2> tin;
         uint32_t tmax = 0;
         for (uint32_t k = 0; k <= uint32_t(n_s); ++k) {
           toff[k] = uint32_t(tin.size());
@@ -570,21 +575,25 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
           for (const auto& t : terms[k - kT0]) {
             if (t.first == sw_vert[k] && t.second == 0) continue;  // the vertex's own value
             if (int64_t(c->isc) * t.second >= (int64_t(1) << 30)) continue;
-            tin.push_back(make_int2(element(t.first), int32_t(t.second)));
-            tin_plain.push_back(make_int2(int32_t(t.first), int32_t(t.second)));
+            tin.push_back(make_int2(plain ? int32_t(t.first) : element(t.first), int32_t(t.second)));
           }
           tmax = std::max(tmax, uint32_t(tin.size()) - toff[k]);
         }
         for (size_t k = size_t(n_s) + 1; k < toff.size(); ++k) toff[k] = uint32_t(tin.size());
         tin.resize(tin.size() + tmax + 2 * kSweepChunk, make_int2(0, 0));
-        upload(c->d_top_off, toff, s);
-        upload(c->d_top_in, tin, s);
-        upload(c->d_top_in_plain, tin_plain, s);
-        c->top_l0 = L0;
-        c->top_maxp = tmax;
+        upload(plain ? c->d_top_off_plain : c->d_top_off, toff, s);
+        upload(plain ? c->d_top_in_plain : c->d_top_in, tin, s);
+        if (plain) {
+          c->top_l0_plain = L0;
+        } else {
+          c->top_l0 = L0;
+          c->top_maxp = tmax;
+        }
         }
       }
-    }
+    };
+    build_top(kTopClosureMax, false);
+    build_top(kTopClosurePlainMax, true);
   }
   c->h_up_off = std::move(up_off);
   c->h_dn_off = std::move(dn_off);
@@ -617,14 +626,15 @@ static void launch_sweeps_t(chsga_ctx* c, int mode) {
                        (const uint32_t*)c->d_sw_vert.p, (const uint32_t*)off, (const int2*)in, k0, k1, c->isc);
     c->launches += 1;
   };
-  if (mode == 3 && c->top_closure && c->top_l0 < c->J) {
-    // levels below top_l0 one by one, the top levels as one closure launch
-    for (int32_t j = 1; j < c->top_l0; ++j) {
+  if (mode == 3 && c->top_closure && c->top_l0_plain < c->J) {
+    // levels below top_l0_plain one by one, the top levels as one closure launch
+    const int32_t jt = c->top_l0_plain;
+    for (int32_t j = 1; j < jt; ++j) {
       const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
       if (c->h_up_off[k1] != c->h_up_off[k0]) level(k0, k1, c->d_up_off.p, c->d_up_in.p);
     }
-    level(c->h_lvl_off[c->top_l0], c->h_lvl_off[c->J], c->d_top_off.p, c->d_top_in_plain.p);
-    for (int32_t j = c->top_l0 - 1; j >= 0; --j) {
+    level(c->h_lvl_off[jt], c->h_lvl_off[c->J], c->d_top_off_plain.p, c->d_top_in_plain.p);
+    for (int32_t j = jt - 1; j >= 0; --j) {
       const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
       if (c->h_dn_off[k1] != c->h_dn_off[k0]) level(k0, k1, c->d_dn_off.p, c->d_dn_in.p);
     }

@@ -60,6 +60,7 @@ constexpr int kSweepChunk = 16;
 constexpr int sweep_record_words(int R) { return R == 2 ? 4 : R == 4 ? 8 : 16; }
 // most vertices of the top CH levels the batched sweep runs as one closure
 constexpr int64_t kTopClosureMax = 16384;
+constexpr int64_t kTopClosurePlainMax = 65536;  // the plain path's closure
 // most {source, weight} terms of that closure; more (long relay chains, as
 // exact closure can give) and the top levels keep their level launches
 constexpr int64_t kTopClosureTermsMax = int64_t(1) << 22;
@@ -127,7 +128,10 @@ struct chsga_ctx {
   bool top_closure = true;  // CHSGA_OPT_TOP_CLOSURE
   uint32_t top_maxp = 0;
   chsga_engine::DevMem d_top_off, d_top_in;
-  chsga_engine::DevMem d_top_in_plain;  // the same lists as string codes (plain sweep: F / Lst)
+  // plain sweep: levels top_l0_plain .. J - 1 as one launch over lists keyed
+  // by string code (F / Lst), from a larger top set (kTopClosurePlainMax)
+  int32_t top_l0_plain = 0;
+  chsga_engine::DevMem d_top_off_plain, d_top_in_plain;
   std::vector<uint8_t> h_up_rec, h_dn_rec;
   std::vector<uint64_t> h_up_rbase, h_dn_rbase;
   TopSet top_plain;
