@@ -633,7 +633,15 @@ static void launch_sweeps_t(chsga_ctx* c, int mode) {
       const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
       if (c->h_up_off[k1] != c->h_up_off[k0]) level(k0, k1, c->d_up_off.p, c->d_up_in.p);
     }
-    level(c->h_lvl_off[jt], c->h_lvl_off[c->J], c->d_top_off_plain.p, c->d_top_in_plain.p);
+    {
+      constexpr int G = 16;  // lanes per top vertex
+      const uint32_t k0 = c->h_lvl_off[jt], k1 = c->h_lvl_off[c->J];
+      const uint64_t threads = uint64_t(k1 - k0) * G;
+      hipLaunchKernelGGL((k_sweep_group<T, G>), dim3(unsigned((threads + CHSGA_BLOCK - 1) / CHSGA_BLOCK)),
+                         dim3(CHSGA_BLOCK), 0, s, F, L, (const uint32_t*)c->d_sw_vert.p,
+                         (const uint32_t*)c->d_top_off_plain.p, (const int2*)c->d_top_in_plain.p, k0, k1, c->isc);
+      c->launches += 1;
+    }
     for (int32_t j = jt - 1; j >= 0; --j) {
       const uint32_t k0 = c->h_lvl_off[j], k1 = c->h_lvl_off[j + 1];
       if (c->h_dn_off[k1] != c->h_dn_off[k0]) level(k0, k1, c->d_dn_off.p, c->d_dn_in.p);

@@ -60,7 +60,7 @@ constexpr int kSweepChunk = 16;
 constexpr int sweep_record_words(int R) { return R == 2 ? 4 : R == 4 ? 8 : 16; }
 // most vertices of the top CH levels the batched sweep runs as one closure
 constexpr int64_t kTopClosureMax = 16384;
-constexpr int64_t kTopClosurePlainMax = 65536;  // the plain path's closure
+constexpr int64_t kTopClosurePlainMax = 262144;  // the plain path's closure
 // most {source, weight} terms of that closure; more (long relay chains, as
 // exact closure can give) and the top levels keep their level launches
 constexpr int64_t kTopClosureTermsMax = int64_t(1) << 22;

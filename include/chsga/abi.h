@@ -139,7 +139,7 @@ typedef struct chsga_stats {
 #define CHSGA_OPT_PACKED_ROWS 10  /* read-only check: 1 if the batched rows are packed (setting a different
                                      value is an error: the choice is made by chsga_ctx_create_ex) */
 #define CHSGA_OPT_TOP_CLOSURE 11  /* 1 (default): the batched sweep and the plain path's full sweeps run the
-                                     top CH levels (at most 16384 vertices batched, 65536 plain) as one launch over precomputed
+                                     top CH levels (at most 16384 vertices batched, 262144 plain) as one launch over precomputed
                                      closure lists; 0: level by level (same results) */
 #define CHSGA_OPT_RING_ROWS 12    /* chsga_traceback_batch, batched backward: at most this many rows in the
                                      ring that pass B recomputes into (-1, default: the plan with the fewest
