@@ -1,10 +1,14 @@
 """A short fixed workload for profilers (rocprofv3 --kernel-trace / --pmc).
 
-    python -m chsga_amd.profile_run [--workload c3] [--rows 300] [--reads 64] [--slots 64] [--mode both|plain|batch]
+    python -m chsga_amd.profile_run [--workload c3] [--rows 300] [--reads 64] [--slots 64]
+                                    [--mode both|plain|batch|traceback] [--k 50]
 
 Takes the first `rows` chars of `reads` reads of the workload and runs read 0
 on the plain path (rows - 1 DP rows) and all of them on the batched path
 (`rows` ticks of `slots` slots, in groups of 64), printing one line per stage.
+`--mode traceback` instead runs the batched traceback of those reads with
+checkpoint rows every `k` rows: every lane stores one on the same ticks
+(rows 0, k, 2k, ...), the most a capture tick can hold.
 """
 from __future__ import annotations
 
@@ -23,7 +27,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=300)
     ap.add_argument("--reads", type=int, default=64)
     ap.add_argument("--slots", type=int, default=64)
-    ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
+    ap.add_argument("--mode", choices=["both", "plain", "batch", "traceback"], default="both")
+    ap.add_argument("--k", type=int, default=50, help="checkpoint interval of --mode traceback")
     a = ap.parse_args()
     w = WORKLOADS[a.workload]
     ds = w.dataset()
@@ -39,6 +44,12 @@ def main() -> int:
         t = time.time()
         s = eng.score_batch(short, slots=a.slots)
         print(f"[profile_run] batched: {len(short)} reads x {a.rows} rows in {time.time() - t:.2f}s", flush=True)
+    if a.mode == "traceback":
+        t = time.time()
+        paths = eng.traceback_batch(short, k=a.k, slots=a.slots)
+        st = eng.last_stats
+        print(f"[profile_run] traceback: {len(short)} reads x {a.rows} rows, k = {a.k}, {st['rows']} ticks, "
+              f"{sum(len(p.ops) for p in paths)} path ops in {time.time() - t:.2f}s", flush=True)
     eng.close()
     return 0
 
