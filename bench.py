@@ -5,7 +5,8 @@ A step is one pass of the hot path over one batch of synthetic reads of the
 c3 workload (20 Mbp seeded chain-of-bubbles DAG, ~22M char vertices; reads
 log-normal median 3 kbp clipped to 1-30 kbp, 2% error, 1% N, mismatchN):
 the optimal score and an alignment path of every read, with checkpoint rows
-every k = 500 kept in HBM.
+every k = 500 kept in HBM.  Step k takes reads k*B .. (k+1)*B - 1 of the
+4096-read c3 stream (cycling), B = --batch-reads.
   value      GCUPS of the batched path with traceback (slot-interleaved rows,
              device work queue, reads packed in order of the learned cost
              model; forward pass keeping every 500th row, then segment
@@ -14,14 +15,24 @@ every k = 500 kept in HBM.
              (--score-only times score-only batches instead);
   plain      GCUPS of the plain path (one read per launch sequence,
              score-only) on the same graph;
-  roofline   the batched row kernel (k_brow_tile, one DP row of 64 reads per
-             launch): algorithmic bytes per launch over its mean duration from
-             HIP events on the engine's stream; traffic = HBM bytes per launch
+  roofline   the batched row kernel (k_brow_packed on c3; k_brow_tile where
+             rows are char rows), one DP row of 64 reads per launch:
+             algorithmic bytes per launch over its mean duration from HIP
+             events on the engine's stream; traffic = HBM bytes per launch
              from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/
              (scaled by a plain copy of known size), when that summary was
              taken on the same per-launch bytes;
+  check      after the timed region: every path of every timed step verified
+             (chsga_host_verify_path: char edges, rescored cost = score, last
+             consuming row = |r| - 1) and the batched scores of a sample of
+             reads against the plain path's;
   cpu_baseline  the oracle (C++/OpenMP restatement of ConSGA) on a bounded
              sample of the same workload on this host's cores.
+
+Wall clock: the run keeps to --time-budget seconds (default 540) from its
+start.  A timed step is started only while the mean step so far, plus the
+time reserved for the legs after the timed region, still fits; `steps`
+reports the steps actually timed (`steps_requested` the ones asked for).
 """
 from __future__ import annotations
 
@@ -46,11 +57,14 @@ def log(*a):
 
 
 def row_kernel_bytes(info: dict, score_bytes: int = 4) -> int:
-    """Algorithmic bytes of one k_row_tile launch (one DP row, one read):
-    read H and write D (2 V scores), labels V, in_off 4(V+1), in_src 4E,
-    first-char bit mask V/8, F and Lst writes (2 n_s scores)."""
-    V, E, n_s = info["V"], info["E"], info["n_s"]
-    return 2 * score_bytes * V + V + 4 * (V + 1) + 4 * E + V // 8 + 2 * score_bytes * n_s
+    """Algorithmic (compulsory) bytes of one compact k_row_tile8 launch (one
+    DP row of one read): read H and write D (2 V scores), labels V, the
+    first-char bit mask V/8, the 8-byte first_nb entry of every string's first
+    char (its first two in-neighbours; the in-CSR is read only for the rare
+    first char with more) and the F and Lst writes (2 n_s scores).  The
+    gathered in-neighbour scores are lines of the same input row."""
+    V, n_s = info["V"], info["n_s"]
+    return 2 * score_bytes * V + V + V // 8 + 8 * n_s + 2 * score_bytes * n_s
 
 
 def batch_kernel_bytes(info: dict, score_bytes: int, slots: int = 64) -> int:
@@ -142,6 +156,16 @@ def row_bytes_survey(info: dict) -> int:
     return 4 * V + 4 * V + V + 4 * (V + 1) + 4 * E + 12 * (info["m_up"] + info["m_dn"]) + 8 * n_s
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
     from oracle import oracle as O
     threads = O.threads(0)
@@ -161,6 +185,7 @@ def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
     gpu_score = gpu_engine.align_combo(sample)
     return {
         "value": round(rows * V / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+        "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
         "sample": f"score-only DP of read 0 of the c3 read set truncated to its first {rows} chars "
                   f"({rows} DP rows x {V} chars; paths are not timed on the CPU), "
                   f"{dt:.1f} s on {threads} OpenMP threads",
@@ -169,6 +194,7 @@ def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
 
 
 def main() -> int:
+    t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -178,10 +204,15 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=500, help="checkpoint interval of the traceback")
     ap.add_argument("--score-only", action="store_true", help="batched steps compute scores only (no paths)")
     ap.add_argument("--plain-reads", type=int, default=2, help="reads per plain step")
+    ap.add_argument("--plain-budget", type=float, default=40.0, help="seconds for the plain leg's timed steps")
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-budget", type=float, default=540.0,
+                    help="wall-clock seconds for the whole run (timed steps stop early to keep it)")
+    ap.add_argument("--check-reads", type=int, default=4,
+                    help="reads of the last timed step whose batched scores are checked against the plain path")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run a torch copy of N MiB (PMC scaling)")
     ap.add_argument("--traceback", type=int, default=0,
                     help="also time the per-read checkpointed traceback of this many reads")
@@ -206,7 +237,11 @@ def main() -> int:
     from chsga_amd import WORKLOADS, Engine
     from chsga_amd import learned as L
     from chsga_amd import native as N
+    from chsga_amd.api import verify_path
     from chsga_amd.scheduler import plan_order
+
+    def elapsed() -> float:
+        return time.perf_counter() - t_start
 
     wl = WORKLOADS[args.workload]
     t = time.perf_counter()
@@ -214,8 +249,8 @@ def main() -> int:
     info = ds.info()
     log(f"[bench] {wl.name} graph: V={info['V']} E={info['E']} n_s={info['n_s']} levels={info['num_layers']} "
         f"({time.perf_counter() - t:.1f}s)")
-    n_needed = max(args.batch_reads, args.plain_reads) * (args.steps + args.warmup)
-    reads = wl.readset(ds, n_reads=n_needed)
+    stream = wl.readset(ds)  # the workload's read stream; steps take consecutive batches, cycling
+    n_stream = len(stream)
     eng = Engine(ds, wl.alphabet)
     eng.set_option(N.OPT_PROFILE, 0)
     # queue order: longest predicted time first, predictions from the learned
@@ -224,34 +259,24 @@ def main() -> int:
     scheduler = "learned cost model (LPT)" if model is not None else "plain formula rows x V (LPT)"
     log(f"[bench] scheduler: {scheduler}")
     V = info["V"]
+    desc = ds.desc(wl.alphabet)
     result = {}
 
     def step_reads(k: int, per: int):
-        return reads.subset(range(k * per, (k + 1) * per))
+        return stream.subset([(k * per + j) % n_stream for j in range(per)])
 
-    # ---- plain path: one read per launch sequence -------------------------
-    plain_gcups = None
-    if args.mode in ("both", "plain"):
-        for w in range(args.warmup):
-            eng.align_combo(step_reads(w, args.plain_reads))
-            log(f"[bench] plain warm-up {w} done")
-        cells = 0
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            rs = step_reads(args.warmup + k, args.plain_reads)
-            eng.align_combo(rs)
-            cells += int(rs.lengths().sum()) * V
-        dt = time.perf_counter() - t0
-        plain_gcups = cells / dt / 1e9
-        result["plain"] = {"gcups": round(plain_gcups, 3), "ms_per_step": round(dt * 1e3 / args.steps, 2),
-                           "reads_per_step": args.plain_reads}
-        log(f"[bench] plain: {plain_gcups:.2f} GCUPS ({dt:.2f}s, {args.steps} steps)")
+    # time reserved for the legs after the batched timed region: plain leg,
+    # roofline profile, path checks, CPU baseline
+    reserve = (args.plain_budget + 15.0 if args.mode in ("both", "plain") else 0.0) + 25.0 + \
+        (0.0 if args.no_cpu else args.cpu_budget + 15.0)
 
     # ---- batched path -------------------------------------------------------
-    # A step is one stream of batch_reads reads through the 64 slots: scores
+    # A step is one batch of batch_reads reads through the 64 slots: scores
     # and paths (checkpointed traceback), or scores only with --score-only;
     # the warm-up is a short batch that allocates the slot buffers.
     batch_gcups = score_gcups = None
+    kept = []  # (reads, scores or paths) of every timed step, checked after the timed region
+    steps_done = 0
     if args.mode in ("both", "batch"):
         for w in range(args.warmup):
             rs = step_reads(w, args.batch_reads)
@@ -259,7 +284,7 @@ def main() -> int:
             eng.score_batch(rs, slots=args.slots, order=plan_order(rs, ds, model))
             if not args.score_only:
                 eng.traceback_batch(rs, k=args.k, order=plan_order(rs, ds, model))
-            log(f"[bench] batched warm-up {w} done")
+            log(f"[bench] batched warm-up {w} done ({elapsed():.0f}s)")
         # batched schedule from the autotuner (setup, outside the timed region)
         tried = eng.autotune(step_reads(0, args.batch_reads), ticks=24)
         best = min(tried, key=lambda t: t["us_per_tick"])
@@ -271,44 +296,77 @@ def main() -> int:
         fwd_s = 0.0
         t0 = time.perf_counter()
         for k in range(args.steps):
+            if k > 0:
+                mean = (time.perf_counter() - t0) / k
+                if elapsed() + mean + reserve > args.time_budget:
+                    log(f"[bench] wall-clock budget: stopping after {k} of {args.steps} timed steps "
+                        f"({elapsed():.0f}s elapsed, {mean:.1f}s per step, {reserve:.0f}s reserved)")
+                    break
             rs = step_reads(args.warmup + k, args.batch_reads)
             ts = time.perf_counter()
             order = plan_order(rs, ds, model)
             if args.score_only:
-                eng.score_batch(rs, slots=args.slots, order=order, use_graphs=True)
+                sc = eng.score_batch(rs, slots=args.slots, order=order, use_graphs=True)
                 fwd_s += time.perf_counter() - ts
+                kept.append((rs, sc))
             else:
                 paths = eng.traceback_batch(rs, k=args.k, slots=args.slots, order=order, use_graphs=True)
                 fwd_s += eng.last_stats["forward_ms"] / 1e3
                 ops += sum(len(p.ops) for p in paths)
+                kept.append((rs, paths))
             st = eng.last_stats
             cells += int(rs.lengths().sum()) * V
             ticks += st["rows"]
             fallbacks += st["walk_fallbacks"]
             peak = max(peak, st["peak_device_bytes"])
-            log(f"[bench] batched step {k}: {len(rs)} reads, {st['rows']} ticks, {time.perf_counter() - ts:.1f}s")
+            steps_done += 1
+            log(f"[bench] batched step {k}: {len(rs)} reads, {st['rows']} ticks, {time.perf_counter() - ts:.1f}s "
+                f"({elapsed():.0f}s elapsed)")
         dt = time.perf_counter() - t0
         batch_gcups = cells / dt / 1e9
         score_gcups = cells / fwd_s / 1e9
-        result["batch"] = {"gcups": round(batch_gcups, 3), "ms_per_step": round(dt * 1e3 / args.steps, 2),
-                           "reads_per_step": args.batch_reads, "slots": args.slots,
+        result["batch"] = {"gcups": round(batch_gcups, 3), "ms_per_step": round(dt * 1e3 / steps_done, 2),
+                           "steps": steps_done, "reads_per_step": args.batch_reads, "slots": args.slots,
                            "work": "scores only" if args.score_only else
                            f"scores and paths (checkpoint rows every {args.k} in HBM)",
                            "score_only_gcups": round(score_gcups, 3), "ticks": ticks}
         if not args.score_only:
             result["batch"].update({"path_ops": ops, "walk_fallbacks": fallbacks,
                                     "peak_device_gb": round(peak / 1e9, 1)})
-        log(f"[bench] batched: {batch_gcups:.2f} GCUPS ({dt:.2f}s); forward pass alone {score_gcups:.2f} GCUPS")
+        log(f"[bench] batched: {batch_gcups:.2f} GCUPS ({dt:.2f}s, {steps_done} steps); "
+            f"forward pass alone {score_gcups:.2f} GCUPS")
+
+    # ---- plain path: one read per launch sequence -------------------------
+    plain_gcups = None
+    plain_steps = 0
+    if args.mode in ("both", "plain"):
+        for w in range(min(args.warmup, 1)):
+            eng.align_combo(step_reads(w, args.plain_reads))
+            log(f"[bench] plain warm-up {w} done")
+        cells = 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            if k > 0 and (time.perf_counter() - t0) * (k + 1) / k > args.plain_budget:
+                break
+            rs = step_reads(args.warmup + k, args.plain_reads)
+            eng.align_combo(rs)
+            cells += int(rs.lengths().sum()) * V
+            plain_steps += 1
+        dt = time.perf_counter() - t0
+        plain_gcups = cells / dt / 1e9
+        result["plain"] = {"gcups": round(plain_gcups, 3), "ms_per_step": round(dt * 1e3 / plain_steps, 2),
+                           "steps": plain_steps, "reads_per_step": args.plain_reads}
+        log(f"[bench] plain: {plain_gcups:.2f} GCUPS ({dt:.2f}s, {plain_steps} steps; {elapsed():.0f}s elapsed)")
 
     # ---- roofline of the dominant kernel (profiled, separate short run) ----
     # HIP events bracket every row-kernel launch on the engine's own stream.
-    score_bytes = 2 if max(reads.lengths()) < 32767 else 4  # int16 storage when provably exact
+    score_bytes = 2 if max(stream.lengths()) < 32767 else 4  # int16 storage when provably exact
     eng.set_option(N.OPT_PROFILE, 1)
-    prof_b = reads.subset(range(min(len(reads), 3 * args.slots)))
+    prof_b = stream.subset(range(min(n_stream, 3 * args.slots)))
     prof_b = type(prof_b).from_list([prof_b[i][:400] for i in range(len(prof_b))])
     eng.score_batch(prof_b, slots=args.slots)
     stb = eng.last_stats
-    eng.align_combo(reads.subset([0]))
+    eng.align_combo(stream.subset([0]))
     stp = eng.last_stats
     eng.set_option(N.OPT_PROFILE, 0)
     b_ms = stb["row_kernel_ms"] / max(1, stb["row_kernel_launches"])
@@ -334,14 +392,44 @@ def main() -> int:
                 "valu": valu, "salu": salu, "kernel": kernel, "kernel_avg_us": round(b_ms * 1e3, 2),
                 "bytes_per_launch": kb, "launches_timed": stb["row_kernel_launches"],
                 "score_storage_bytes": score_bytes,
-                "plain_kernel": {"kernel": "k_row_tile", "avg_us": round(p_ms * 1e3, 2), "bytes_per_launch": pb,
+                "plain_kernel": {"kernel": "k_row_tile8", "avg_us": round(p_ms * 1e3, 2), "bytes_per_launch": pb,
                                  "achieved_GBps": round(pb / (p_ms * 1e-3) / 1e9, 1),
                                  "sweep_post_avg_us_per_row": round(stp["sweep_ms"] * 1e3 /
                                                                     max(1, stp["row_kernel_launches"]), 2)},
                 "batched_sweep_post_avg_us_per_tick": round(stb["sweep_ms"] * 1e3 /
                                                             max(1, stb["row_kernel_launches"]), 2)}
     log(f"[bench] {kernel}: {b_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
-        f"k_row_tile: {p_ms * 1e3:.1f} us/launch")
+        f"k_row_tile8: {p_ms * 1e3:.1f} us/launch")
+
+    # ---- checks of the timed work (outside the timed region) ---------------
+    check = None
+    if kept:
+        t1 = time.perf_counter()
+        n_paths = bad_paths = 0
+        first_bad = None
+        for rs, res in kept:
+            if args.score_only:
+                break
+            for i, p in enumerate(res):
+                rc, at = verify_path(desc, rs[i], p.ops, p.score)
+                n_paths += 1
+                if rc != 0:
+                    bad_paths += 1
+                    first_bad = first_bad or {"read": i, "code": rc, "op": at}
+        rs, res = kept[-1]
+        sc = np.array([p.score for p in res] if not args.score_only else res, dtype=np.int64)
+        # a sample of the last step's reads: the shortest ones and an evenly spread few
+        lens = rs.lengths()
+        pick = sorted(set(np.argsort(lens, kind="stable")[:max(1, args.check_reads // 2)].tolist() +
+                          np.linspace(0, len(rs) - 1, max(1, args.check_reads - args.check_reads // 2),
+                                      dtype=int).tolist()))
+        plain = eng.align_combo(rs.subset(pick)).astype(np.int64)
+        check = {"paths_verified": n_paths, "paths_failed": bad_paths, "first_failure": first_bad,
+                 "scores_checked_against_plain": len(pick), "read_lengths_checked": lens[pick].tolist(),
+                 "scores_equal_plain": bool(np.array_equal(plain, sc[pick])),
+                 "scores_in_bounds": bool(((sc >= 0) & (sc <= lens)).all()),
+                 "seconds": round(time.perf_counter() - t1, 1)}
+        log(f"[bench] checks: {check}")
 
     # ---- optional: the per-read checkpointed traceback, for comparison ----
     if args.traceback > 0:
@@ -356,19 +444,23 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(ds, wl, reads, args.cpu_budget, eng)
-        log(f"[bench] cpu baseline: {cpu['value']} GCUPS on {cpu['cores']} threads, exact={cpu['exact']}")
+        cpu = cpu_baseline(ds, wl, stream, args.cpu_budget, eng)
+        log(f"[bench] cpu baseline: {cpu['value']} GCUPS on {cpu['cores']} threads ({cpu['cpu_model']}), "
+            f"exact={cpu['exact']}")
 
     value = batch_gcups if batch_gcups is not None else plain_gcups
-    ms = result.get("batch", result.get("plain"))["ms_per_step"]
+    main_leg = result.get("batch", result.get("plain"))
+    n_reads = args.batch_reads if batch_gcups is not None else args.plain_reads
     line = {
-        "metric": METRIC, "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": main_leg["steps"],
+        "steps_requested": args.steps, "warmup": args.warmup, "ms_per_step": main_leg["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded graph and reads)",
-        "config": {"workload": f"{wl.name}: 20 Mbp seeded DAG (SNP 1/25, indels 1-50 at 1/200), V={V} chars, "
-                               f"n_s={info['n_s']}; reads log-normal median 3 kbp (1-30 kbp), 2% error, 1% N, "
-                               f"mismatchN, " + ("scores only" if args.score_only else
-                                                 f"scores and paths, checkpoint rows every {args.k} in HBM"),
+        "config": {"workload": f"{n_reads} reads of the {wl.name} stream per step: 20 Mbp seeded DAG (SNP 1/25, "
+                               f"indels 1-50 at 1/200), V={V} chars, n_s={info['n_s']}; reads log-normal median "
+                               f"3 kbp (1-30 kbp), 2% error, 1% N, mismatchN, " +
+                               ("scores only" if args.score_only or batch_gcups is None else
+                                f"scores and paths, checkpoint rows every {args.k} in HBM"),
                    "batch_reads": args.batch_reads, "plain_reads": args.plain_reads, "slots": args.slots,
                    "scheduler": scheduler, "schedule": result.get("schedule")},
         "autotune": result.get("autotune"),
@@ -376,8 +468,8 @@ def main() -> int:
         "batched_gcups": None if batch_gcups is None else round(batch_gcups, 3),
         "score_only_gcups": None if score_gcups is None else round(score_gcups, 3),
         "speedup_batched_vs_plain": (round(batch_gcups / plain_gcups, 2) if batch_gcups and plain_gcups else None),
-        "batch": result.get("batch"),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "batch": result.get("batch"), "plain": result.get("plain"), "check": check,
+        "roofline": roofline, "cpu_baseline": cpu, "wall_s": round(elapsed(), 1),
     }
     if "traceback_per_read" in result:
         line["traceback_per_read"] = result["traceback_per_read"]

@@ -203,6 +203,21 @@ class TracebackResult:
 OP_DTYPE = np.dtype([("row", np.int64), ("vertex", np.uint32), ("op", np.uint8)], align=True)
 
 
+def verify_path(desc: N.GraphDesc, read: bytes, ops: np.ndarray, score: int) -> tuple:
+    """The drivers' path verification (src/ConSGA_traceback.cpp:184-225,
+    chsga_host_verify_path): (0, -1) when the path follows char edges and
+    rescores to `score` with its last consuming row at len - 1, else
+    (code, index of the failing op)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    bad = C.c_int64(-1)
+    rc = N.host_lib().chsga_host_verify_path(C.byref(desc), read, len(read),
+                                             ops.ctypes.data_as(C.POINTER(N.AlignOp)), len(ops), int(score),
+                                             C.byref(bad))
+    if rc < 0:
+        N._check_host(rc)
+    return rc, bad.value
+
+
 class Engine:
     """A chsga_ctx on one GPU."""
 

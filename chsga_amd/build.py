@@ -71,11 +71,18 @@ def build(verbose: bool = False) -> None:
     rpath = "-Wl,-rpath,$ORIGIN/../lib"
     drv = CSRC / "drivers"
     stage2 = []
-    for name, src, defs in [
-        ("ConSGA_mismatchN_dag", "consga.cpp", ["-DCHSGA_ALPHABET=5"]),
-        ("ConSGA_matchN_dag", "consga.cpp", ["-DCHSGA_ALPHABET=4"]),
-        ("ConSGA_traceback_mismatchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=5"]),
-        ("ConSGA_traceback_matchN_dag", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=4"]),
+    # The reference's ConSGA targets (Makefile:121-124, 142-145, 156-171): DAG
+    # is unused by ConSGA (src/config.hpp:28-40) and the _800gb variant only
+    # raises the cap of a full score-matrix store this engine does not keep
+    # (its checkpoints live in HBM with overflow to pinned host memory), so
+    # each alphabet builds one program under every name.
+    drivers = []
+    for alpha, sz in (("mismatchN", 5), ("matchN", 4)):
+        for suffix in ("_dag", ""):
+            drivers.append((f"ConSGA_{alpha}{suffix}", "consga.cpp", [f"-DCHSGA_ALPHABET={sz}"]))
+            drivers.append((f"ConSGA_traceback_{alpha}{suffix}", "consga_traceback.cpp", [f"-DCHSGA_ALPHABET={sz}"]))
+    drivers.append(("ConSGA_traceback_mismatchN_dag_800gb", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=5"]))
+    for name, src, defs in drivers + [
         ("build_pch", "build_pch.cpp", []),
         ("gfa_to_csr", "gfa_to_csr.cpp", []),
         ("csr_to_adj", "csr_to_adj.cpp", []),

@@ -995,6 +995,67 @@ int chsga_host_save_reads(const chsga_readset* rs, const char* path) {
   })
 }
 
+// The verification of src/ConSGA_traceback.cpp:184-225 (path rescored with
+// the edit costs, last consuming row = |r| - 1), plus the structure the
+// rescoring assumes: the path opens with the row-0 MATCHSUB that
+// align_read_with_traceback appends (src/traceback.hpp:1020-1027), and every
+// step moves along a char edge (MATCHSUB, INSERT) or stays on its char
+// (DELETE) with the row advancing as the op says.
+int chsga_host_verify_path(const chsga_graph_desc* g, const char* read, int64_t len, const chsga_align_op* ops,
+                           int64_t n_ops, int32_t score, int64_t* bad_index) {
+  if (!g || !read || len <= 0 || !g->labels || !g->in_off || !g->in_src) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  int64_t bad = 0;
+  int code = 0;
+  auto edge = [&](uint32_t u, uint32_t v) {
+    const uint32_t* a = g->in_src + g->in_off[v];
+    const uint32_t* b = g->in_src + g->in_off[v + 1];
+    return std::find(a, b, u) != b;
+  };
+  auto sig = [&](char a, char b) {
+    const bool m = g->alphabet == CHSGA_ALPHABET_MATCH_N ? (a == 'N' || b == 'N' || a == b) : a == b;
+    return m ? g->match_score : g->subst_score;
+  };
+  int64_t cost = 0, consumed = -1;
+  if (n_ops <= 0 || !ops) {
+    code = 1;
+  } else if (ops[0].row != 0 || ops[0].op != CHSGA_OP_MATCHSUB) {
+    code = 2;
+  } else {
+    for (int64_t j = 0; j < n_ops && !code; ++j) {
+      const chsga_align_op& o = ops[j];
+      bad = j;
+      if (int64_t(o.vertex) >= g->V || o.row < 0 || o.row >= len || o.op > CHSGA_OP_INSERT) {
+        code = 4;
+        break;
+      }
+      if (j > 0) {
+        const chsga_align_op& p = ops[j - 1];
+        const bool step = o.op == CHSGA_OP_DELETE     ? o.row == p.row + 1 && o.vertex == p.vertex
+                          : o.op == CHSGA_OP_MATCHSUB ? o.row == p.row + 1 && edge(p.vertex, o.vertex)
+                                                      : o.row == p.row && edge(p.vertex, o.vertex);
+        if (!step) {
+          code = 3;
+          break;
+        }
+      }
+      if (o.op == CHSGA_OP_INSERT) {
+        cost += g->ins_score;
+      } else {
+        cost += o.op == CHSGA_OP_DELETE ? g->del_score : sig(g->labels[o.vertex], read[o.row]);
+        consumed = o.row;
+      }
+    }
+    if (!code && consumed != len - 1) code = 5;
+    if (!code && cost != score) code = 6;
+    if (code >= 5) bad = n_ops - 1;
+  }
+  if (bad_index) *bad_index = code ? bad : -1;
+  return code;
+}
+
 int64_t chsga_reads_count(const chsga_readset* rs) { return rs ? int64_t(rs->off.size()) - 1 : 0; }
 const char* chsga_reads_data(const chsga_readset* rs) { return rs ? rs->data.data() : nullptr; }
 const int64_t* chsga_reads_offsets(const chsga_readset* rs) { return rs ? rs->off.data() : nullptr; }

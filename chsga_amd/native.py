@@ -165,6 +165,8 @@ HOST_SYMBOLS = {
     "chsga_reads_free": (None, [C.c_void_p]),
     "chsga_ds_labels": (C.c_void_p, [C.c_void_p]),
     "chsga_ds_str_off": (C.c_void_p, [C.c_void_p]),
+    "chsga_host_verify_path": (C.c_int, [C.POINTER(GraphDesc), C.c_char_p, C.c_int64, C.POINTER(AlignOp), C.c_int64,
+                                         C.c_int32, i64p]),
 }
 
 _libs: dict[str, C.CDLL] = {}

@@ -115,6 +115,17 @@ const char* chsga_reads_data(const chsga_readset* rs);
 const int64_t* chsga_reads_offsets(const chsga_readset* rs);
 void chsga_reads_free(chsga_readset* rs);
 
+/* Path verification <- src/ConSGA_traceback.cpp:184-225 (rescoring of the
+ * path with the edit costs, last consuming row = len - 1), with the
+ * structure it assumes also checked.  Returns 0 when the path is valid and
+ * rescores to `score`, else the first failed check: 1 empty path, 2 first op
+ * not (row 0, MATCHSUB), 3 a step that does not follow a char edge or
+ * advances the row wrongly, 4 vertex / row / op out of range, 5 last
+ * consuming row != len - 1, 6 cost != score; CHSGA_EINVAL on null
+ * arguments.  *bad_index (optional) names the failing op (-1 when valid). */
+int chsga_host_verify_path(const chsga_graph_desc* g, const char* read, int64_t len, const chsga_align_op* ops,
+                           int64_t n_ops, int32_t score, int64_t* bad_index);
+
 /* Direct array access for tests and tools (pointers into the dataset). */
 const char* chsga_ds_labels(const chsga_dataset* ds);
 const uint64_t* chsga_ds_str_off(const chsga_dataset* ds);

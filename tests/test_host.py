@@ -226,3 +226,20 @@ def test_engine_rejects_bad_descriptor_without_touching_the_gpu():
     d.alphabet = 5
     d.ins_score = -1
     assert lib.chsga_ctx_create(ctypes.byref(d), 0, ctypes.byref(h)) == -1
+
+
+def test_drivers_under_every_reference_name():
+    """The ConSGA targets of the reference's Makefile (:121-124, :142-145,
+    :156-171) are built; each rejects an unknown flag with its usage line and
+    a missing input with an error, before any GPU call."""
+    import subprocess
+    bin_ = ROOT / "chsga_amd" / "bin"
+    names = [f"ConSGA_{a}{s}" for a in ("matchN", "mismatchN") for s in ("", "_dag")] + \
+        [f"ConSGA_traceback_{a}{s}" for a in ("matchN", "mismatchN") for s in ("", "_dag")] + \
+        ["ConSGA_traceback_mismatchN_dag_800gb"]
+    for n in names:
+        assert (bin_ / n).exists(), n
+        bad = subprocess.run([str(bin_ / n), "-a", "x"], capture_output=True, text=True, timeout=60)
+        assert bad.returncode != 0 and "Usage" in bad.stderr, n
+        miss = subprocess.run([str(bin_ / n), "-I", "x"], capture_output=True, text=True, timeout=60)
+        assert miss.returncode != 0 and "missing required arguments" in miss.stderr, n
