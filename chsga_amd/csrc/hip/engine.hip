@@ -903,6 +903,14 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
       case CHSGA_OPT_RING_ROWS:
         ctx->ring_rows_cap = value < 0 ? -1 : value;
         break;
+      case CHSGA_OPT_FORWARD_TAIL:
+        if (value < 0) throw std::invalid_argument("CHSGA_OPT_FORWARD_TAIL must be >= 0");
+        ctx->forward_tail = value;
+        break;
+      case CHSGA_OPT_BACKWARD_TAIL:
+        if (value < 0) throw std::invalid_argument("CHSGA_OPT_BACKWARD_TAIL must be >= 0");
+        ctx->backward_tail = value;
+        break;
       case CHSGA_OPT_PACKED_ROWS:
         if ((value != 0) != ctx->packed)
           throw std::invalid_argument(ctx->packed ? "the batched rows are packed"

@@ -143,6 +143,8 @@ struct chsga_ctx {
   int64_t ckpt_hbm_rows = -1;     // per-read traceback: checkpoint rows kept in HBM (-1: all that fit)
   int64_t seg_rows_cap = -1;      // per-read traceback: segment store rows (-1: by free memory)
   int64_t ring_rows_cap = -1;     // batched backward: ring rows at most (-1: by free memory)
+  int64_t backward_tail = 6;      // batched backward: tail read by read (0: never)
+  int64_t forward_tail = 3;       // batched forward: tail on the plain path (0: never)
   bool compact = false;  // char graph is the expansion of the string graph
   bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)
   int64_t launches = 0, rows = 0;

@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from chsga_amd import Engine, ReadSet, WORKLOADS
+from chsga_amd import native as N
 from chsga_amd.api import verify_path
 from chsga_amd.scheduler import plan_order, simulate_ticks
 from oracle import oracle as O
@@ -87,6 +88,7 @@ def test_c4_slice_learned_and_formula_orders():
     model = L.load_default()
     assert model is not None, "the trained cost model is committed under chsga_amd/data"
     eng = Engine(ds, w.alphabet)
+    eng.set_option(N.OPT_FORWARD_TAIL, 0)  # every tick batched: the tick count is the queue's
     res = {}
     for name, m in (("learned", model), ("formula", None)):
         order = plan_order(rs, ds, m)
@@ -100,6 +102,9 @@ def test_c4_slice_learned_and_formula_orders():
         assert ticks == simulate_ticks(rs.lengths(), order)
     assert np.array_equal(res["learned"][0], res["formula"][0])
     sc = res["learned"][0]
+    eng.set_option(N.OPT_FORWARD_TAIL, 3)
+    _say("c4 learned order with the forward tail on the plain path")
+    assert np.array_equal(eng.score_batch(rs, order=res["learned"][3], use_graphs=True), sc)
     lens = rs.lengths()
     assert (sc >= 0).all() and (sc <= lens).all()
     sub = sorted(set(np.argsort(lens, kind="stable")[:3].tolist() + [100, 700]))

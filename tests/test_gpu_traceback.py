@@ -182,10 +182,23 @@ def test_batched_backward_equals_per_read_backward(k, ring):
     eng.set_option(N.OPT_NARROW, 1)
     print("batched backward, two slot groups and two backward workers", k, flush=True)
     d = eng.traceback_batch(rs, k=k, slots=128)
+    # the read-by-read tail: never (rounds to the end), and from the first
+    # round on (every read's segments on the plain path, checkpoints from the
+    # batched pool, int16 and int32)
+    eng.set_option(N.OPT_BACKWARD_TAIL, 0)
+    print("batched backward without the read-by-read tail", k, flush=True)
+    e = eng.traceback_batch(rs, k=k)
+    eng.set_option(N.OPT_BACKWARD_TAIL, 10 ** 6)
+    print("batched forward, every segment read by read", k, flush=True)
+    f = eng.traceback_batch(rs, k=k)
+    eng.set_option(N.OPT_NARROW, 0)
+    g = eng.traceback_batch(rs, k=k, slots=128)
     for i in range(len(rs)):
         assert a[i].score == b[i].score == c[i].score == d[i].score
         assert np.array_equal(a[i].ops, b[i].ops) and np.array_equal(a[i].ops, c[i].ops), (k, i)
         assert np.array_equal(a[i].ops, d[i].ops), (k, i)
+        for o in (e, f, g):
+            assert o[i].score == a[i].score and np.array_equal(o[i].ops, a[i].ops), (k, i)
     eng.close()
 
 
@@ -320,3 +333,38 @@ def test_exact_closure_matches_the_exact_oracle(tmp_path):
                     rows_differ += int((row != O.consga_last_row(d, rs[i])).any())
             eng.close()
     assert rows_differ > 0
+
+
+@pytest.mark.parametrize("packed", [True, False])
+def test_forward_tail_on_the_plain_path(packed):
+    """Once every read has started and few slots stay busy, the batched
+    passes finish the running reads one by one on the plain path: scores
+    equal the oracle's and every-tick batching, and the batched traceback's
+    checkpoint rows (stored from the plain path for those reads) give the
+    same paths, with int16 and int32 rows."""
+    ds, rs = small_workload(seed=47, n_reads=70, length=1400, error=0.06, n_rate=0.02)
+    # two long reads and many short ones: the tail holds the long ones
+    reads = ReadSet.from_list([rs[0], rs[1][:1100]] + [rs[i][: 20 + (i * 37) % 300] for i in range(2, 70)])
+    ref = O.consga_scores(ds.desc(), reads)
+    eng = Engine(ds, packed=packed)
+    out = {}
+    for tail in (0, 3, 64):
+        eng.set_option(N.OPT_FORWARD_TAIL, tail)
+        print("forward tail", packed, tail, flush=True)
+        sc = eng.score_batch(reads, use_graphs=True)
+        assert np.array_equal(sc, ref), tail
+        ticks = eng.last_stats["rows"]
+        for narrow in (1, 0):
+            eng.set_option(N.OPT_NARROW, narrow)
+            out[tail, narrow] = eng.traceback_batch(reads, k=97, use_graphs=True)
+        eng.set_option(N.OPT_NARROW, 1)
+        if tail == 64:
+            assert ticks > 0
+    d = ds.desc()
+    base = out[0, 1]
+    for key, o in out.items():
+        for i in range(len(reads)):
+            assert o[i].score == ref[i] and np.array_equal(o[i].ops, base[i].ops), (key, i)
+    for i in (0, 1, 5):
+        assert O.validate_path(d, reads[i], base[i].ops, base[i].score)[0] == 0
+    eng.close()
