@@ -115,6 +115,26 @@ static void build_top(chsga_ctx* c, TopSet& T, int32_t j_top, const std::vector<
   T.n_dni = uint32_t(dni.size());
 }
 
+// Scratch of chsga_score_reads_noalloc: CHSGA_NOALLOC_MAX_READS argmin keys
+// followed by as many int32 scores (1.5 * 65536 * 8 bytes = 768 KiB), held by
+// every GPU the library runs on until the process ends.  Host code never
+// passes the array's name to a kernel: each device's own copy is reached
+// through the address hipGetSymbolAddress gives on that device.
+static __device__ __align__(256) unsigned long long scratch[CHSGA_NOALLOC_MAX_READS + CHSGA_NOALLOC_MAX_READS / 2];
+
+// Address of `scratch` on the current device, looked up once per device.
+static void* noalloc_scratch(int device) {
+  static std::mutex mu;
+  static std::unordered_map<int, void*> addr;
+  const std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void* p = nullptr;
+  hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  addr[device] = p;
+  return p;
+}
+
 static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   if (!g || g->V <= 0 || !g->labels || !g->in_off || !g->in_src || g->n_s <= 0 || !g->str_off || !g->rank ||
       !g->layer_off || !g->cc_off || !g->up_off || !g->dn_off)
@@ -292,6 +312,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
 
   // every check above is host-only; the device is touched from here on
   hip_check(hipSetDevice(c->device), "hipSetDevice");
+  c->d_scratch = noalloc_scratch(c->device);  // with the context's device current
   hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   hipStream_t s = c->stream;
   std::vector<uint8_t> labels(g->labels, g->labels + V);
@@ -977,6 +998,44 @@ int chsga_score_reads(chsga_ctx* ctx, const char* const* reads, const int64_t* l
       stats->kernel_launches = ctx->launches;
       stats->rows = ctx->rows;
     }
+  })
+}
+
+int chsga_score_reads_noalloc(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,
+                              int32_t* scores_out) {
+  if (!ctx || (n > 0 && (!reads || !lens || !scores_out)) || n < 0) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  if (n > CHSGA_NOALLOC_MAX_READS) {
+    set_error("chsga_score_reads_noalloc: more than CHSGA_NOALLOC_MAX_READS (65536) reads in one call");
+    return CHSGA_EINVAL;
+  }
+  ENGINE_GUARD({
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->d_scratch) throw std::logic_error("no-allocation scratch not set up for this context");
+    for (int64_t r = 0; r < n; ++r)
+      if (lens[r] <= 0 || !reads[r]) throw std::invalid_argument("empty read");
+    if (n == 0) return CHSGA_OK;
+    // the device's scratch array: argmin keys, then the int32 scores
+    unsigned long long* keys = static_cast<unsigned long long*>(ctx->d_scratch);
+    int32_t* sc = reinterpret_cast<int32_t*>(keys + CHSGA_NOALLOC_MAX_READS);
+    const bool prof = ctx->profile;
+    ctx->profile = false;
+    hipLaunchKernelGGL(k_fill_u64, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, keys, n, ~0ull);
+    for (int64_t r = 0; r < n; ++r) {
+      ctx->narrow = ctx->fits_narrow(lens[r]);
+      const int cur = ctx->forward(reads[r], lens[r], 0, nullptr);
+      ctx->launch_argmin(ctx->d_row[cur].p, keys + r);
+      ctx->narrow = false;
+    }
+    ctx->profile = prof;
+    hipLaunchKernelGGL(k_key_scores, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const unsigned long long*)keys, sc, n);
+    hip_check(hipMemcpyAsync(scores_out, sc, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, ctx->stream),
+              "copy scores");
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    hip_check(hipGetLastError(), "kernel launch");
   })
 }
 

@@ -137,6 +137,7 @@ struct chsga_ctx {
   TopSet top_plain;
   // working rows
   chsga_engine::DevMem d_F, d_Lst, d_row[2];
+  void* d_scratch = nullptr;  // this device's copy of the no-allocation scratch array
   // options and counters
   bool profile = false, use_graphs = false, force_staged = false;
   bool batched_backward = true;  // batched traceback: lock-step segment recompute when it fits

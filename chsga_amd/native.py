@@ -123,6 +123,7 @@ ENGINE_SYMBOLS = {
     "chsga_set_batch_schedule": (C.c_int, [C.c_void_p, C.POINTER(BatchSchedule)]),
     "chsga_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, C.c_int32,
                                        C.POINTER(TuneResult), C.c_int32, C.POINTER(C.c_int32)]),
+    "chsga_score_reads_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, i32p]),
     "chsga_traceback_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_int64, C.POINTER(C.POINTER(Path_)),
                                        C.POINTER(Stats)]),
     "chsga_path_free": (None, [C.POINTER(Path_)]),

@@ -188,6 +188,18 @@ int chsga_score_reads(chsga_ctx* ctx, const char* const* reads, const int64_t* l
                       int32_t* scores_out, double* total_sec, double* sub_sec, double* ins_sec,
                       chsga_stats* stats);
 
+/* No counterpart in the reference.  chsga_score_reads for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same plain path and scores,
+ * with the per-call argmin buffer in a static device array of the library
+ * (768 KiB per GPU, kept until the process ends) instead of a hipMalloc, so
+ * the call allocates nothing.  At most CHSGA_NOALLOC_MAX_READS reads per call
+ * (more: CHSGA_EINVAL).  Every context on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two host threads, two contexts) race. */
+#define CHSGA_NOALLOC_MAX_READS 65536
+int chsga_score_reads_noalloc(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n,
+                              int32_t* scores_out);
+
 /* Last DP row of one read (V int32 values) copied to row_out; for tests. */
 int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_out);
 
