@@ -77,11 +77,15 @@ def test_gpu_training_fits_a_known_function():
 
 
 @pytest.mark.gpu
-def test_trained_model_gains_over_scheduling_without_it():
-    """The committed model (trained on the GPU on logged plain runs): its
-    time predictions rank reads as their lengths do, so LPT packing by the
-    model needs fewer ticks than FIFO packing without a model; and its error
-    on the logged tuples is reported next to the plain formula's."""
+def test_trained_model_against_the_formula_and_fifo():
+    """The committed model (trained on the GPU on logged plain runs) against
+    the method without it.  On one graph a read's DP cost is rows x V plus a
+    per-row launch term, both linear in its length, so the model and the
+    plain formula rank reads alike and their LPT orders pack the c4 stream
+    in the same number of ticks: the model gains nothing over its fallback
+    there (both beat FIFO).  What it adds is the time of a read across
+    graphs: its held-out median error is below the fitted formula's."""
+    import json
     m = L.load_default()
     if m is None:
         pytest.skip("no trained model committed yet")
@@ -91,5 +95,9 @@ def test_trained_model_gains_over_scheduling_without_it():
     rs = w.readset(ds, n_reads=2048)
     fifo = simulate_ticks(rs.lengths(), fifo_order(rs), 64)
     learned = simulate_ticks(rs.lengths(), plan_order(rs, ds, m), 64)
-    assert learned < fifo
+    formula = simulate_ticks(rs.lengths(), plan_order(rs, ds, None), 64)
+    print("ticks: fifo", fifo, "formula", formula, "learned", learned, flush=True)
+    assert learned == formula < fifo
+    rep = json.loads((L.DATA_DIR / "cost_model_report.json").read_text())
+    assert rep["test_median_rel_err_time"] < rep["formula_median_rel_err_time"]
     m.close()
