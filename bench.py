@@ -213,6 +213,11 @@ def main() -> int:
                     help="wall-clock seconds for the whole run (timed steps stop early to keep it)")
     ap.add_argument("--check-reads", type=int, default=4,
                     help="reads of the last timed step whose batched scores are checked against the plain path")
+    ap.add_argument("--forward-tail", type=int, default=None,
+                    help="CHSGA_OPT_FORWARD_TAIL (busy slots on average below which a batch's tail runs on the plain path)")
+    ap.add_argument("--backward-tail", type=int, default=None,
+                    help="CHSGA_OPT_BACKWARD_TAIL (ratio below which the backward's last reads go read by read)")
+    ap.add_argument("--first-step", type=int, default=0, help="index of the first timed step's batch in the stream")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run a torch copy of N MiB (PMC scaling)")
     ap.add_argument("--traceback", type=int, default=0,
                     help="also time the per-read checkpointed traceback of this many reads")
@@ -253,6 +258,10 @@ def main() -> int:
     n_stream = len(stream)
     eng = Engine(ds, wl.alphabet)
     eng.set_option(N.OPT_PROFILE, 0)
+    if args.forward_tail is not None:
+        eng.set_option(N.OPT_FORWARD_TAIL, args.forward_tail)
+    if args.backward_tail is not None:
+        eng.set_option(N.OPT_BACKWARD_TAIL, args.backward_tail)
     # queue order: longest predicted time first, predictions from the learned
     # cost model (trained on the GPU, chsga_amd/data) or the plain formula
     model = L.load_default()
@@ -302,7 +311,7 @@ def main() -> int:
                     log(f"[bench] wall-clock budget: stopping after {k} of {args.steps} timed steps "
                         f"({elapsed():.0f}s elapsed, {mean:.1f}s per step, {reserve:.0f}s reserved)")
                     break
-            rs = step_reads(args.warmup + k, args.batch_reads)
+            rs = step_reads(args.warmup + args.first_step + k, args.batch_reads)
             ts = time.perf_counter()
             order = plan_order(rs, ds, model)
             if args.score_only:

@@ -3,8 +3,13 @@
 An FP32 MLP (8 inputs -> 64 -> 64 -> 2, ReLU), trained on the GPU by the
 library's own kernels (chsga_mlp_* in include/chsga/abi.h), predicts for a
 read on a graph:
-  out[0] = log(seconds of the plain score-only alignment of the read)
+  out[0] = log(seconds per DP row of the plain score-only alignment of the
+           read; times its rows, the read's seconds)
   out[1] = log(peak device bytes of its checkpointed traceback with interval k)
+The time is learned per row, so that a read's predicted time grows with its
+length outside the lengths the model was trained on (a network predicting
+whole-read seconds from short reads does not extrapolate monotonically to
+reads of 30 kbp, and its queue order then packs worse than the formula's).
 from features of the read and the graph.  The predictions drive
   * packing: the batched queue takes reads longest-predicted-time first;
   * HBM admission and k: the traceback interval is the largest k whose
@@ -120,7 +125,7 @@ class CostModel:
 
     # --- scheduling -------------------------------------------------------
     def predict_time(self, reads, ds, k: int = 500) -> np.ndarray:
-        return np.exp(self.forward(read_features(reads, ds.info(), k))[:, 0])
+        return np.exp(self.forward(read_features(reads, ds.info(), k))[:, 0]) * reads.lengths().astype(np.float64)
 
     def predict_bytes(self, reads, ds, k: int) -> np.ndarray:
         return np.exp(self.forward(read_features(reads, ds.info(), k))[:, 1])

@@ -6,7 +6,8 @@
    seeded reads with log-uniform lengths and varying N fractions.
 2. Every read is aligned on the plain path; its measured seconds (HIP stream
    synchronised per read) and the peak device bytes of its checkpointed
-   traceback at a drawn k form one tuple.  The bytes of a sample are measured
+   traceback at a drawn k form one tuple.  The network learns the time per
+   DP row (seconds / rows), so its predictions extrapolate to longer reads.  The bytes of a sample are measured
    by running the traceback and must equal traceback_peak_bytes().
 3. The MLP is trained on the GPU (chsga_mlp_train) on log targets and saved;
    its error is reported next to the plain formula's (a * rows * V + b * rows,
@@ -91,25 +92,38 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--tuples", type=int, default=8000)
     ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=str(ROOT / "chsga_amd" / "data"))
+    ap.add_argument("--from-tuples", default=None,
+                    help="train on tuples logged before (a cost_tuples.npz) instead of logging new runs")
     args = ap.parse_args()
     from chsga_amd import learned as L
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
-    X, T, meta, checked = collect(args.tuples)
+    if args.from_tuples:
+        with np.load(args.from_tuples, allow_pickle=False) as z:
+            X, T, meta = z["X"], z["T"], z["meta"]
+        checked = int(json.loads((Path(args.from_tuples).parent / "cost_model_report.json").read_text())
+                      .get("bytes_measurements_checked", 0)) if (Path(args.from_tuples).parent /
+                                                                  "cost_model_report.json").exists() else 0
+    else:
+        X, T, meta, checked = collect(args.tuples)
     np.savez(out / "cost_tuples.npz", X=X, T=T, meta=meta)
     n = len(X)
+    # the time target per DP row (a read of len chars runs len rows)
+    Tt = T.copy()
+    Tt[:, 0] = T[:, 0] - np.log(meta[:, 1])
     rng = np.random.default_rng(0)
     perm = rng.permutation(n)
     tr, te = perm[: int(0.8 * n)], perm[int(0.8 * n):]
     m = L.CostModel(seed=11)
-    loss = m.fit(X[tr], T[tr], epochs=args.epochs, batch=256, lr=2e-3, seed=5)
+    loss = m.fit(X[tr], Tt[tr], epochs=args.epochs, batch=256, lr=args.lr, seed=5)
     pred = m.forward(X[te])
-    t_err = float(np.median(np.abs(np.exp(pred[:, 0] - T[te, 0]) - 1)))
+    t_err = float(np.median(np.abs(np.exp(pred[:, 0] - Tt[te, 0]) - 1)))
     b_err = float(np.median(np.abs(np.exp(pred[:, 1] - T[te, 1]) - 1)))
     f_err = formula_error(meta[te])
     m.save(out / "cost_model.npz")
-    report = {"tuples": n, "train_loss": loss, "test_median_rel_err_time": t_err,
+    report = {"tuples": n, "epochs": args.epochs, "lr": args.lr, "train_loss": loss, "test_median_rel_err_time": t_err,
               "test_median_rel_err_bytes": b_err, "formula_median_rel_err_time": f_err,
               "bytes_measurements_checked": checked}
     (out / "cost_model_report.json").write_text(json.dumps(report, indent=1) + "\n")

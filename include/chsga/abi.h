@@ -145,10 +145,13 @@ typedef struct chsga_stats {
                                      ring that pass B recomputes into (-1, default: the plan with the fewest
                                      recomputed rows whose buffers fit in free memory; never fewer than
                                      ceil(sqrt(k)) rows) */
-#define CHSGA_OPT_BACKWARD_TAIL 13 /* chsga_traceback_batch, batched backward: once the segments the reads
-                                     still walking have left, summed, are at most this many times the most
-                                     any one of them has left, those reads finish read by read on the plain
-                                     path (same paths; default 6; 0: never) */
+#define CHSGA_OPT_BACKWARD_TAIL 13 /* chsga_traceback_batch, batched backward: the cost of a round of 64
+                                     segments in segments recomputed read by read (default 6; 0: every read
+                                     in rounds).  Before each round, all reads still walking finish read by
+                                     read on the plain path when their summed segments are at most this
+                                     many rounds' worth, else the read with the most segments left does when
+                                     the rounds it alone keeps going cost more than its segments (same
+                                     paths) */
 #define CHSGA_OPT_FORWARD_TAIL 14  /* batched passes (chsga_score_batch, chsga_traceback_batch's forward): once
                                      every read has started and the rows the running reads have left are at
                                      most this many times the ticks left (on average that many busy slots),

@@ -341,7 +341,9 @@ def test_forward_tail_on_the_plain_path(packed):
     passes finish the running reads one by one on the plain path: scores
     equal the oracle's and every-tick batching, and the batched traceback's
     checkpoint rows (stored from the plain path for those reads) give the
-    same paths, with int16 and int32 rows."""
+    same paths, with int16 and int32 rows; in the backward the two long
+    reads leave the rounds and finish read by read (against rounds to the
+    end)."""
     ds, rs = small_workload(seed=47, n_reads=70, length=1400, error=0.06, n_rate=0.02)
     # two long reads and many short ones: the tail holds the long ones
     reads = ReadSet.from_list([rs[0], rs[1][:1100]] + [rs[i][: 20 + (i * 37) % 300] for i in range(2, 70)])
@@ -350,6 +352,9 @@ def test_forward_tail_on_the_plain_path(packed):
     out = {}
     for tail in (0, 3, 64):
         eng.set_option(N.OPT_FORWARD_TAIL, tail)
+        # the backward: rounds to the end against the default, where the two
+        # long reads leave the rounds read by read
+        eng.set_option(N.OPT_BACKWARD_TAIL, 0 if tail == 0 else 6)
         print("forward tail", packed, tail, flush=True)
         sc = eng.score_batch(reads, use_graphs=True)
         assert np.array_equal(sc, ref), tail
