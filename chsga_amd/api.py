@@ -222,19 +222,22 @@ class Engine:
     """A chsga_ctx on one GPU."""
 
     def __init__(self, ds: Dataset, alphabet: int = N.ALPHABET_MISMATCH_N, device: int = 0, desc=None,
-                 packed: bool = True, exact: bool = False):
+                 packed: bool = True, exact: bool = False, cut: bool = False):
         """desc: a GraphDesc of ds to use instead of ds.desc(alphabet), e.g.
         with other edit costs (the context copies what it needs).  packed:
         let the batched path store rows as 2-bit deltas where that is exact
         (compact graphs, unit costs); False keeps one score per char.
         exact: the exact insertion closure of every row (ParSGA's
-        semantics) instead of ConSGA's stale last values."""
+        semantics) instead of ConSGA's stale last values.  cut: exact, with
+        the plain path closing rows on match-cut string components
+        (ParSGA's cut-graph insertion) instead of the CH sweep."""
         self.ds = ds
         self.alphabet = alphabet if desc is None else int(desc.alphabet)
         self._lib = N.engine_lib()
         h = C.c_void_p()
         d = ds.desc(alphabet) if desc is None else desc
-        flags = (0 if packed else N.CREATE_CHAR_ROWS) | (N.CREATE_EXACT_CLOSURE if exact else 0)
+        flags = ((0 if packed else N.CREATE_CHAR_ROWS) | (N.CREATE_EXACT_CLOSURE if exact else 0)
+                 | (N.CREATE_CUT_CLOSURE if cut else 0))
         N._check_engine(self._lib.chsga_ctx_create_ex(C.byref(d), device, flags, C.byref(h)))
         self._h = h
         self.last_stats: dict = {}

@@ -117,10 +117,12 @@ inline void load_inputs(const char* I, const char* i, const char* c, const char*
   chsga_host_describe(in.ds, CHSGA_ALPHABET, &in.desc);
 }
 
-// exact: ParSGA's semantics (CHSGA_CREATE_EXACT_CLOSURE) instead of ConSGA's
-inline void open_engine(Inputs& in, int device, bool exact = false) {
-  if (chsga_ctx_create_ex(&in.desc, device, exact ? CHSGA_CREATE_EXACT_CLOSURE : 0u, &in.ctx) != CHSGA_OK)
-    die("Error: ", chsga_last_error());
+// closure 'E': ParSGA's semantics (CHSGA_CREATE_EXACT_CLOSURE) instead of
+// ConSGA's; 'X': the same values, the plain path closing rows on match-cut
+// components (CHSGA_CREATE_CUT_CLOSURE); 0: ConSGA's
+inline void open_engine(Inputs& in, int device, char closure = 0) {
+  const uint32_t flags = closure == 'X' ? CHSGA_CREATE_CUT_CLOSURE : closure == 'E' ? CHSGA_CREATE_EXACT_CLOSURE : 0u;
+  if (chsga_ctx_create_ex(&in.desc, device, flags, &in.ctx) != CHSGA_OK) die("Error: ", chsga_last_error());
 }
 
 }  // namespace drv

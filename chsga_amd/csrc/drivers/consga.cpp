@@ -7,6 +7,8 @@
 //   -B          batched path (slot-interleaved rows, device work queue)
 //   -g <dev>    GPU index
 //   -E          exact insertion closure (ParSGA's scores) instead of ConSGA's
+//   -X          the same scores, rows closed on match-cut components (ParSGA's
+//               cut-graph insertion) on the plain path
 #include "common.hpp"
 
 int main(int argc, char** argv) {
@@ -14,10 +16,10 @@ int main(int argc, char** argv) {
        *offsets_f = nullptr;
   int64_t cap = 0;
   bool batched = false;
-  bool exact = false;
+  char closure = 0;
   int device = 0;
   int opt;
-  while ((opt = getopt(argc, argv, "I:C:i:c:r:l:o:a:s:N:M:y:n:BEg:")) != -1) {
+  while ((opt = getopt(argc, argv, "I:C:i:c:r:l:o:a:s:N:M:y:n:BEXg:")) != -1) {
     switch (opt) {
       case 'I': orig_char = optarg; break;
       case 'i': orig_str = optarg; break;
@@ -27,7 +29,8 @@ int main(int argc, char** argv) {
       case 'o': offsets_f = optarg; break;
       case 'n': cap = std::atoll(optarg); break;
       case 'B': batched = true; break;
-      case 'E': exact = true; break;
+      case 'E':
+      case 'X': closure = char(opt); break;
       case 'g': device = std::atoi(optarg); break;
       default:
         std::cerr << "Usage: " << argv[0]
@@ -47,7 +50,7 @@ int main(int argc, char** argv) {
   drv::Inputs in;
   double ch_time = 0;
   drv::load_inputs(orig_char, orig_str, contr, labels_f, offsets_f, in, &ch_time);
-  drv::open_engine(in, device, exact);
+  drv::open_engine(in, device, closure);
   std::cout << "input ch time: " << ch_time << '\n';
   std::cout << "input time: " << drv::now_s() - input_start << '\n' << std::flush;
 

@@ -6,7 +6,8 @@
 // Extra flags: -n <cap> (read cap, 0 = all), -g <dev> (GPU index), -B (batched
 // forward pass keeping every k-th row of every read, then per-read segment
 // recompute and walk: chsga_traceback_batch), -E (exact insertion closure:
-// ParSGA's scores and paths instead of ConSGA's).
+// ParSGA's scores and paths instead of ConSGA's), -X (the same, rows closed
+// on match-cut components on the plain path: ParSGA's cut-graph insertion).
 #include "common.hpp"
 
 // print_mean_with_moe of src/ConSGA_traceback.cpp:26-51 (z = 1.96)
@@ -32,14 +33,16 @@ int main(int argc, char** argv) {
        *offsets_f = nullptr;
   int64_t k = 500, cap = 0;
   std::string dir = "/tmp/consga_traceback_ckpts";
-  bool benchmark = false, batched = false, exact = false;
+  bool benchmark = false, batched = false;
+  char closure = 0;
   int device = 0;
   int opt;
-  while ((opt = getopt(argc, argv, "bBEI:i:c:r:l:o:k:d:n:g:")) != -1) {
+  while ((opt = getopt(argc, argv, "bBEXI:i:c:r:l:o:k:d:n:g:")) != -1) {
     switch (opt) {
       case 'b': benchmark = true; break;
       case 'B': batched = true; break;
-      case 'E': exact = true; break;
+      case 'E':
+      case 'X': closure = char(opt); break;
       case 'I': orig_char = optarg; break;
       case 'i': orig_str = optarg; break;
       case 'c': contr = optarg; break;
@@ -70,7 +73,7 @@ int main(int argc, char** argv) {
   drv::load_reads(reads_f, cap, reads);
   drv::Inputs in;
   drv::load_inputs(orig_char, orig_str, contr, labels_f, offsets_f, in, nullptr);
-  drv::open_engine(in, device, exact);
+  drv::open_engine(in, device, closure);
   const int match_score = in.desc.match_score, subst_score = in.desc.subst_score, del_score = in.desc.del_score,
             ins_score = in.desc.ins_score;
   const size_t n = reads.size();

@@ -69,6 +69,8 @@ static int64_t now_ns() {
 using namespace chsga_engine;
 using namespace chsga_dev;
 
+#include "cut.hip.inc"
+
 // ---------------------------------------------------------------------------
 // context set-up
 // ---------------------------------------------------------------------------
@@ -309,6 +311,8 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   if (c->top_level_opt >= 0) c->j_top = std::max(c->j_top, std::min<int32_t>(int32_t(c->top_level_opt), J));
   std::vector<uint32_t> pos_of_og(n_s);
   for (int64_t og = 0; og < n_s; ++og) pos_of_og[og] = kpos[g->rank[og]];
+
+  if (c->cut) build_cut_tables(c);  // host only: it may still refuse the graph or the costs
 
   // every check above is host-only; the device is touched from here on
   hip_check(hipSetDevice(c->device), "hipSetDevice");
@@ -714,7 +718,7 @@ void chsga_ctx::launch_row(const void* H, void* D, uint8_t rc) {
     launch_row_t<int32_t>(this, H, D, rc);
   if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
   launches += 1;
-  launch_insertion(D);
+  launch_insertion(D, rc);
   if (profile) hip_check(hipEventRecord(ev_pool[ev_used++], s), "event");
 }
 
@@ -725,7 +729,8 @@ void chsga_ctx::launch_sweeps(int mode) {
     launch_sweeps_t<int32_t>(this, mode);
 }
 
-void chsga_ctx::launch_insertion(void* D) {
+void chsga_ctx::launch_insertion(void* D, uint8_t rc) {
+  if (cut_on && (narrow ? launch_cut_t<int16_t>(this, D, rc) : launch_cut_t<int32_t>(this, D, rc))) return;
   launch_sweeps(3);
   if (narrow)
     launch_post_t<int16_t>(this, D);
@@ -861,14 +866,19 @@ int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, c
   *out = nullptr;
   chsga_ctx* c = nullptr;
   ENGINE_GUARD({
-    if (flags & ~uint32_t(CHSGA_CREATE_CHAR_ROWS | CHSGA_CREATE_EXACT_CLOSURE))
+    if (flags & ~uint32_t(CHSGA_CREATE_CHAR_ROWS | CHSGA_CREATE_EXACT_CLOSURE | CHSGA_CREATE_CUT_CLOSURE))
       throw std::invalid_argument("unknown context flags");
     c = new chsga_ctx();
     try {
       c->device = device;
       c->allow_packed = (flags & CHSGA_CREATE_CHAR_ROWS) == 0;
-      c->exact = (flags & CHSGA_CREATE_EXACT_CLOSURE) != 0;
+      c->cut = (flags & CHSGA_CREATE_CUT_CLOSURE) != 0;
+      c->exact = c->cut || (flags & CHSGA_CREATE_EXACT_CLOSURE) != 0;
       build_context(c, g);
+      if (c->cut) {
+        upload_cut_tables(c, c->stream);  // build_context leaves the context's device current
+        c->cut_on = true;
+      }
     } catch (...) {
       delete c;
       throw;
@@ -931,6 +941,11 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
       case CHSGA_OPT_BACKWARD_TAIL:
         if (value < 0) throw std::invalid_argument("CHSGA_OPT_BACKWARD_TAIL must be >= 0");
         ctx->backward_tail = value;
+        break;
+      case CHSGA_OPT_CUT_CLOSURE:
+        if (value != 0 && !ctx->cut)
+          throw std::invalid_argument("no cut tables: create the context with CHSGA_CREATE_CUT_CLOSURE");
+        ctx->cut_on = value != 0;
         break;
       case CHSGA_OPT_PACKED_ROWS:
         if ((value != 0) != ctx->packed)

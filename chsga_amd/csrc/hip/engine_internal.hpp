@@ -73,6 +73,20 @@ struct TopSet {
   chsga_engine::DevMem ext_off, ext, upi_off, upi, dni_off, dni;
 };
 
+// cut closure (CHSGA_CREATE_CUT_CLOSURE): read chars A, C, G, T, N
+constexpr int kCutClasses = 5;
+// largest component (strings) one thread of the cut closure relaxes; a class
+// with a larger one keeps the CH sweep
+constexpr uint32_t kCutMaxStrings = 64;
+constexpr uint32_t kCutWindow = 65536;  // chars per window of the cut closure's component order
+struct CutClass {
+  bool ok = false;  // the class runs the cut closure
+  uint32_t n_items = 0, n_comp = 0, largest = 0;
+  std::vector<uint4> h_items;  // until uploaded
+  std::vector<uint32_t> h_comp_off;
+  chsga_engine::DevMem items, comp_off;
+};
+
 struct OgEdge {
   uint32_t v;
   int32_t w;
@@ -106,6 +120,9 @@ struct chsga_ctx {
   bool allow_packed = true;  // chsga_ctx_create_ex without CHSGA_CREATE_CHAR_ROWS
   bool exact = false;        // CHSGA_CREATE_EXACT_CLOSURE: live last values in the sweep
   bool packed = false;       // batched rows packed as 2-bit deltas (batch_fast, unit costs)
+  bool cut = false;          // CHSGA_CREATE_CUT_CLOSURE: cut tables built
+  bool cut_on = false;       // CHSGA_OPT_CUT_CLOSURE: the plain path runs them
+  CutClass cut_cls[kCutClasses];
   std::vector<int4> h_pbnb;        // packed: per string {string, length - 1} of its two in-neighbour strings
   std::vector<uint8_t> h_lclass;   // packed: match class of each char's label
   chsga_engine::DevMem d_cstr;  // packed: string of each char (batched traceback walk)
@@ -163,7 +180,7 @@ struct chsga_ctx {
   ~chsga_ctx();
   void launch_init(void* H, uint8_t c0);
   void launch_row(const void* H, void* D, uint8_t rc);
-  void launch_insertion(void* D);
+  void launch_insertion(void* D, uint8_t rc);
   void launch_argmin(const void* D, unsigned long long* key);
   bool fits_narrow(int64_t max_len) const;
   void launch_sweeps(int mode);  // bit 0: up levels, bit 1: down levels

@@ -2,13 +2,17 @@
 
     python -m chsga_amd.profile_run [--workload c3] [--rows 300] [--reads 64] [--slots 64]
                                     [--mode both|plain|batch|traceback] [--k 50]
+                                    [--closure consga|exact|cut]
 
 Takes the first `rows` chars of `reads` reads of the workload and runs read 0
 on the plain path (rows - 1 DP rows) and all of them on the batched path
 (`rows` ticks of `slots` slots, in groups of 64), printing one line per stage.
 `--mode traceback` instead runs the batched traceback of those reads with
 checkpoint rows every `k` rows: every lane stores one on the same ticks
-(rows 0, k, 2k, ...), the most a capture tick can hold.
+(rows 0, k, 2k, ...), the most a capture tick can hold.  `--closure` picks
+the rows' insertion closure: ConSGA's (default), the exact closure by the CH
+sweep (CHSGA_CREATE_EXACT_CLOSURE) or by match-cut components on the plain
+path (CHSGA_CREATE_CUT_CLOSURE).
 """
 from __future__ import annotations
 
@@ -29,12 +33,13 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--mode", choices=["both", "plain", "batch", "traceback"], default="both")
     ap.add_argument("--k", type=int, default=50, help="checkpoint interval of --mode traceback")
+    ap.add_argument("--closure", choices=["consga", "exact", "cut"], default="consga")
     a = ap.parse_args()
     w = WORKLOADS[a.workload]
     ds = w.dataset()
     rs = w.readset(ds, n_reads=a.reads)
     short = ReadSet.from_list([rs[i][: a.rows] for i in range(len(rs))])
-    eng = Engine(ds, w.alphabet)
+    eng = Engine(ds, w.alphabet, exact=a.closure == "exact", cut=a.closure == "cut")
     print(f"[profile_run] {a.workload}: {ds.info()}", flush=True)
     if a.mode in ("both", "plain"):
         t = time.time()

@@ -157,6 +157,9 @@ typedef struct chsga_stats {
                                      most this many times the ticks left (on average that many busy slots),
                                      with 256 ticks or more left, those reads finish one by one on the plain
                                      path (same scores, argmins and checkpoint rows; default 3; 0: never) */
+#define CHSGA_OPT_CUT_CLOSURE 15   /* 1 (default on contexts created with CHSGA_CREATE_CUT_CLOSURE): the
+                                     plain path closes rows on the match-cut components; 0: with the CH
+                                     sweep (same values).  1 on other contexts: CHSGA_EINVAL */
 
 const char* chsga_last_error(void);
 int chsga_device_count(int* count);
@@ -181,6 +184,20 @@ int chsga_ctx_create(const chsga_graph_desc* g, int device, chsga_ctx** out);
  * ConSGA's.  Scores are exact int32 (the reference's ParSGA drivers keep
  * uint8 scores, which wrap above 255). */
 #define CHSGA_CREATE_EXACT_CLOSURE 2u
+/* CHSGA_CREATE_CUT_CLOSURE: CHSGA_CREATE_EXACT_CLOSURE's values, and the
+ * plain path computes each row's closure on match-cut string components
+ * instead of the CH sweep: ParSGA's cut-graph insertion
+ * (src/ParSGAoverrides/csr_align.hpp:422-452, cut graphs per read base as
+ * read by src/ParSGAoverrides/read_csr.hpp:50-98), which drops the edges
+ * into chars whose label matches the row's read char and relaxes each weakly
+ * connected component on its own, in topological order.  Needs edit costs
+ * with match <= subst, match <= del + ins and ins >= subst - match (where the
+ * cut is exact; unit costs qualify), a char graph that expands an acyclic
+ * string graph, else CHSGA_EINVAL.  Rows whose read char is not one of
+ * A C G T N, or whose class has a component of more than 64 strings (N under
+ * mismatchN on graphs without N labels), and the batched paths, keep the CH
+ * sweep: the values are the same. */
+#define CHSGA_CREATE_CUT_CLOSURE 4u
 int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, chsga_ctx** out);
 void chsga_ctx_destroy(chsga_ctx* ctx);
 int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value);

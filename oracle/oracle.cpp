@@ -165,6 +165,32 @@ struct Model {
     }
     std::memcpy(last_row, a.data(), sizeof(int32_t) * V);
   }
+
+  // ParSGA's cut-graph insertion, serially: row i relaxes only the edges of
+  // the match-cut graph of r[i], whose edge u -> v survives iff L[v] is
+  // neither 'N' nor r[i] (src/ParSGAoverrides/preprocessing.hpp:122-126),
+  // in a topological order (the DAG branch of insertion,
+  // src/ParSGAoverrides/csr_align.hpp:436-447, per component; the serial
+  // order relaxes the same edges with every dependency before it).  Under
+  // matchN with read char N nothing is relaxed (csr_align.hpp:584-590).
+  void run_cut(const char* read, int64_t len, int32_t* last_row, const std::vector<uint32_t>& order) const {
+    const int64_t V = g->V;
+    std::vector<int32_t> a(V), b(V);
+    init_row(a.data(), read[0]);
+    for (int64_t i = 1; i < len; ++i) {
+      del_sub(b.data(), a.data(), read[i]);
+      const bool skip = g->alphabet == CHSGA_ALPHABET_MATCH_N && read[i] == 'N';
+      for (uint32_t v : order) {
+        if (skip || g->labels[v] == 'N' || g->labels[v] == read[i]) continue;
+        for (uint64_t e = g->in_off[v]; e < g->in_off[v + 1]; ++e) {
+          const int32_t c = b[g->in_src[e]] + g->ins_score;
+          if (c < b[v]) b[v] = c;
+        }
+      }
+      std::swap(a, b);
+    }
+    std::memcpy(last_row, a.data(), sizeof(int32_t) * V);
+  }
 };
 
 bool check_desc(const chsga_graph_desc* g) {
@@ -227,6 +253,15 @@ int oracle_exact_last_row(const chsga_graph_desc* g, const char* read, int64_t l
   const auto order = m.topo();
   if (int64_t(order.size()) != g->V) return CHSGA_EINVAL;  // not a DAG
   m.run_exact(read, len, row_out, order);
+  return CHSGA_OK;
+}
+
+int oracle_cut_last_row(const chsga_graph_desc* g, const char* read, int64_t len, int32_t* row_out) {
+  if (!check_desc(g) || !read || len <= 0 || !row_out) return CHSGA_EINVAL;
+  Model m(g);
+  const auto order = m.topo();
+  if (int64_t(order.size()) != g->V) return CHSGA_EINVAL;
+  m.run_cut(read, len, row_out, order);
   return CHSGA_OK;
 }
 

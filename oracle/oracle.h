@@ -39,6 +39,10 @@ int oracle_consga_rows(const chsga_graph_desc* g, const char* read, int64_t len,
 int oracle_consga_last_row(const chsga_graph_desc* g, const char* read, int64_t len, int32_t* row_out);
 
 int oracle_exact_last_row(const chsga_graph_desc* g, const char* read, int64_t len, int32_t* row_out);
+/* ParSGA's match-cut insertion, serially (edges into chars labelled N or the
+ * row's read char dropped); equals oracle_exact_last_row where the cut is
+ * exact. */
+int oracle_cut_last_row(const chsga_graph_desc* g, const char* read, int64_t len, int32_t* row_out);
 int oracle_exact_scores(const chsga_graph_desc* g, const char* reads, const int64_t* off, int64_t n,
                         int32_t* scores);
 

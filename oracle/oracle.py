@@ -29,6 +29,7 @@ def lib() -> C.CDLL:
         L.oracle_consga_rows.argtypes = [G, C.c_char_p, C.c_int64, N.i32p]
         L.oracle_consga_last_row.argtypes = [G, C.c_char_p, C.c_int64, N.i32p]
         L.oracle_exact_last_row.argtypes = [G, C.c_char_p, C.c_int64, N.i32p]
+        L.oracle_cut_last_row.argtypes = [G, C.c_char_p, C.c_int64, N.i32p]
         L.oracle_exact_scores.argtypes = [G, C.c_char_p, N.i64p, C.c_int64, N.i32p]
         L.oracle_traceback.restype = C.c_int64
         L.oracle_traceback.argtypes = [G, C.c_char_p, C.c_int64, C.POINTER(N.AlignOp), C.c_int64, N.i32p]
@@ -82,6 +83,12 @@ def consga_last_row(desc: N.GraphDesc, read: bytes) -> np.ndarray:
 def exact_last_row(desc: N.GraphDesc, read: bytes) -> np.ndarray:
     out = np.zeros(desc.V, np.int32)
     _ok(lib().oracle_exact_last_row(C.byref(desc), read, len(read), out.ctypes.data_as(N.i32p)), "exact row")
+    return out
+
+
+def cut_last_row(desc: N.GraphDesc, read: bytes) -> np.ndarray:
+    out = np.zeros(desc.V, np.int32)
+    _ok(lib().oracle_cut_last_row(C.byref(desc), read, len(read), out.ctypes.data_as(N.i32p)), "cut row")
     return out
 
 

@@ -299,6 +299,15 @@ def test_drop_in_drivers(tmp_path):
                          capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr
     assert "[RESULT] ALL CHECKS PASSED" in out.stdout
+    # -X: the same scores, rows closed on match-cut components (plain path)
+    out = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-X"], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.splitlines()[-1] == "score: " + " ".join(str(int(x)) for x in exact)
+    out = subprocess.run([str(bin_ / "ConSGA_traceback_mismatchN_dag"), *args, "-k", "100", "-X"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    assert "[RESULT] ALL CHECKS PASSED" in out.stdout
     bad = subprocess.run([str(bin_ / "ConSGA_mismatchN_dag"), *args, "-a", "x"], capture_output=True, text=True)
     assert bad.returncode != 0 and "Usage" in bad.stderr
 
