@@ -200,6 +200,36 @@ def test_c5_prefixes_against_oracle(pool):
     eng.close()
 
 
+@pytest.mark.timeout(900)
+def test_c5_full_length_long_read():
+    """c5 at the workload's own read length: the first long read of the
+    stream (34,376 chars of the 5-50 kbp half, up to 5% error) on the 116.7M
+    char graph.  Its score (int32 rows: the bound passes 2^15) is at most
+    |r|; the per-read traceback (k = 500) with its checkpoint rows in pinned
+    host memory and the two-level segment store, as the workload's traceback
+    store spills, gives a path that validates and rescores to that score; an
+    error-free 6 kbp read sampled from the graph scores 0."""
+    w = WORKLOADS["c5"]
+    ds = w.dataset()
+    rs = w.readset(ds, n_reads=8)
+    i = int(np.argmax(rs.lengths()))
+    assert rs.lengths()[i] >= 5000
+    clean = ReadSet.generate(ds, seed=11, n_reads=1, length=6000)
+    reads = ReadSet.from_list([rs[i], clean[0]])
+    d = ds.desc(w.alphabet)
+    eng = Engine(ds, w.alphabet)
+    _say("c5 plain", reads.lengths().tolist())
+    s = eng.align_combo(reads)
+    assert 0 <= s[0] <= len(reads[0]) and s[1] == 0
+    eng.set_option(N.OPT_CKPT_HBM_ROWS, 0)
+    eng.set_option(N.OPT_SEG_ROWS, 64)
+    _say("c5 scores", s.tolist(), "traceback k=500, host checkpoints, two-level segments")
+    tb = eng.align_read_with_traceback(reads[0], k=500)
+    assert eng.last_stats["host_bytes"] > 0
+    _check_paths(d, reads.subset([0]), [tb], s[:1])
+    eng.close()
+
+
 @pytest.mark.timeout(600)
 def test_gfa_chain_through_the_drivers_against_oracle(tmp_path):
     """A graph entering as GFA goes through the reference's tool chain:
