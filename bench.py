@@ -114,8 +114,8 @@ def packed_kernel_bytes(ds, tile_chars: int, score_bytes: int, slots: int = 64) 
     """Algorithmic (compulsory) bytes of one k_brow_packed launch (packed
     rows: 2-bit codes per char and slot, F/PL/PF planes per string).  Read
     and written once each: the delta words of every tile (whole 16-char
-    groups per tile, 4 bytes per slot), planes F and PF of every string and
-    plane PL of every string of more than one char (score bytes per slot);
+    groups per tile, 4 bytes per slot), plane F of every string and planes
+    PF and PL of every string of more than one char (score bytes per slot);
     read once: the 128-byte table entry of every row-pass chunk (char count,
     string starts, label classes, gathered planes).  Gathered in-neighbour
     planes are the same planes of the input row."""
@@ -123,7 +123,7 @@ def packed_kernel_bytes(ds, tile_chars: int, score_bytes: int, slots: int = 64) 
     lens = np.diff(a["str_off"].astype(np.int64))
     groups, chunks = packed_layout(ds, tile_chars)
     n_s, n_multi = len(lens), int((lens > 1).sum())
-    return 2 * groups * slots * 4 + 2 * (2 * n_s + n_multi) * slots * score_bytes + 128 * chunks
+    return 2 * groups * slots * 4 + 2 * (n_s + 2 * n_multi) * slots * score_bytes + 128 * chunks
 
 
 def profile_summary(kernel: str, bytes_per_launch: int) -> dict:
