@@ -4,6 +4,7 @@
   Engine.align_combo                align_combo (src/align.hpp:470-531): exact int32 scores
   Engine.align_read_with_traceback  align_read_with_traceback (src/traceback.hpp:588-1031)
   Engine.score_batch                batched align_combo over a read stream
+  PasgalAligner.align               PaSGAL_match's alignToDAGLocal (src/PaSGALoverrides/align.hpp:500-700)
 """
 from __future__ import annotations
 
@@ -54,6 +55,13 @@ class Dataset:
         h = C.c_void_p()
         N._check_host(N.host_lib().chsga_host_load_string_csr(str(path).encode(), C.byref(h)))
         return cls(h.value)
+
+    def reverse(self, seed: int = 1) -> "Dataset":
+        """The reversed graph (char j becomes V - 1 - j) with its own CH, for
+        PaSGAL's phase 1-R (chsga_host_reverse)."""
+        h = C.c_void_p()
+        N._check_host(N.host_lib().chsga_host_reverse(self._h, seed, C.byref(h)))
+        return Dataset(h.value)
 
     def save(self, prefix: str) -> None:
         N._check_host(N.host_lib().chsga_host_save(self._h, str(prefix).encode()))
@@ -385,3 +393,40 @@ class Engine:
 
     def __del__(self):
         self.close()
+
+
+class PasgalAligner:
+    """PaSGAL_match, the reference's baseline local aligner
+    (src/PaSGALoverrides/align.hpp), on the GPU: two exact-closure contexts,
+    one on the char DAG and one on its reversal (chsga_pasgal_align)."""
+
+    def __init__(self, ds: Dataset, device: int = 0, score_only: bool = False, seed: int = 1):
+        self.fwd = Engine(ds, device=device, exact=True)
+        self.rev_ds = None if score_only else ds.reverse(seed)
+        self.rev = None if score_only else Engine(self.rev_ds, device=device, exact=True)
+        self.last_stats: dict = {}
+
+    def align(self, reads: ReadSet, match: int = 1, mismatch: int = 1, ins: int = 1, dele: int = 1,
+              score_only: bool = False) -> list:
+        """One dict per read: score, strand, qry_start/end, ref_start/end (char
+        ids), both strands' phase-1 results, rev_best, flags and the compact
+        CIGAR (None when score_only)."""
+        n = len(reads)
+        buf, arr, lens = reads.ptrs()
+        prm = N.PasgalParams(match, mismatch, ins, dele, 1 if score_only else 0)
+        res = (N.PasgalResult * max(n, 1))()
+        st = N.Stats()
+        lib = self.fwd._lib
+        rev_h = self.rev._h if self.rev is not None else None
+        N._check_engine(lib.chsga_pasgal_align(self.fwd._h, rev_h, C.byref(prm), arr, lens.ctypes.data_as(N.i64p), n,
+                                               res, C.byref(st)))
+        self.last_stats = st.as_dict()
+        try:
+            return [res[i].as_dict() for i in range(n)]
+        finally:
+            lib.chsga_pasgal_free(res, n)
+
+    def close(self) -> None:
+        self.fwd.close()
+        if self.rev is not None:
+            self.rev.close()

@@ -2,7 +2,7 @@
 
   chsga_amd/lib/libchsga_host.so   host formats, generator, CH builder (g++)
   chsga_amd/lib/libchsga.so        gfx950 HIP engine behind include/chsga/abi.h (hipcc)
-  chsga_amd/bin/*                  drop-in drivers (ConSGA_*, ConSGA_traceback_*, build_pch, chsga_gen)
+  chsga_amd/bin/*                  drop-in drivers (ConSGA_*, ConSGA_traceback_*, PaSGAL_match, build_pch, ...)
   oracle/build/liboracle.so        CPU restatement used as the checker (g++, OpenMP)
 
 Run `python -m chsga_amd.build` (or __graft_entry__.build()).
@@ -64,7 +64,7 @@ def build(verbose: bool = False) -> None:
          "-o", LIB / "libchsga_host.so"],
         [HIPCC, *HIP_FLAGS, "-shared", "-I", INC, "-I", CSRC / "hip", hip_src, "-lpthread",
          "-o", LIB / "libchsga.so"],
-        [CXX, *CXX_FLAGS, "-fopenmp", "-shared", "-I", INC, ORACLE / "oracle.cpp",
+        [CXX, *CXX_FLAGS, "-fopenmp", "-shared", "-I", INC, ORACLE / "oracle.cpp", ORACLE / "pasgal.cpp",
          "-o", ORACLE / "build" / "liboracle.so"],
     ]
     _run(stage1, verbose)
@@ -82,6 +82,8 @@ def build(verbose: bool = False) -> None:
             drivers.append((f"ConSGA_{alpha}{suffix}", "consga.cpp", [f"-DCHSGA_ALPHABET={sz}"]))
             drivers.append((f"ConSGA_traceback_{alpha}{suffix}", "consga_traceback.cpp", [f"-DCHSGA_ALPHABET={sz}"]))
     drivers.append(("ConSGA_traceback_mismatchN_dag_800gb", "consga_traceback.cpp", ["-DCHSGA_ALPHABET=5"]))
+    # the reference's baseline (Makefile:197-213: PaSGAL with the override align.hpp)
+    drivers.append(("PaSGAL_match", "pasgal.cpp", ["-DCHSGA_ALPHABET=5"]))
     for name, src, defs in drivers + [
         ("build_pch", "build_pch.cpp", []),
         ("gfa_to_csr", "gfa_to_csr.cpp", []),

@@ -638,7 +638,7 @@ static void launch_row_t(chsga_ctx* c, const void* H, void* D, uint8_t rc) {
                      (const uint8_t*)c->d_labels.p, (const uint32_t*)c->d_in_off.p, (const uint32_t*)c->d_in_src.p,
                      (const int2*)c->d_first_nb.p, (const uint32_t*)c->d_first_bits.p,
                      (const uint32_t*)c->d_tile_char.p, (const uint32_t*)c->d_tile_str.p, c->V, rc, c->alphabet,
-                     c->msc, c->ssc, c->dsc, c->isc);
+                     c->msc, c->ssc, c->dsc, c->isc, LocalRow{});
 }
 
 template <typename T>
@@ -1078,3 +1078,4 @@ int chsga_last_row(chsga_ctx* ctx, const char* read, int64_t len, int32_t* row_o
 #include "batch.hip.inc"
 #include "traceback.hip.inc"
 #include "mlp.hip.inc"
+#include "pasgal.hip.inc"

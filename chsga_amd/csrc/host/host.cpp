@@ -803,6 +803,46 @@ int chsga_host_load_string_csr(const char* path, chsga_dataset** out) {
   })
 }
 
+// Reversal for PaSGAL's phase 1-R (src/PaSGALoverrides/align.hpp:146-245
+// walks chars V-1..0 over out-edges): string s becomes n_s - 1 - s with its
+// label reversed, edge u -> v becomes v' -> u', so char j becomes V - 1 - j;
+// then the char expansion and a CH of the reversed string graph.
+int chsga_host_reverse(const chsga_dataset* ds, uint64_t seed, chsga_dataset** out) {
+  if (!ds || !out) {
+    set_error("null argument");
+    return CHSGA_EINVAL;
+  }
+  CHSGA_GUARD({
+    const int64_t n = ds->n_s;
+    auto r = std::make_unique<chsga_dataset>();
+    r->n_s = n;
+    r->str_off.assign(n + 1, 0);
+    r->labels.reserve(ds->labels.size());
+    for (int64_t t = 0; t < n; ++t) {
+      const int64_t s = n - 1 - t;
+      const uint64_t a = ds->str_off[s], b = ds->str_off[s + 1];
+      for (uint64_t c = b; c > a; --c) r->labels.push_back(ds->labels[c - 1]);
+      r->str_off[t + 1] = r->labels.size();
+    }
+    std::vector<std::vector<uint32_t>> adj(n);
+    for (int64_t u = 0; u < n; ++u)
+      for (uint64_t e = ds->str_out_off[u]; e < ds->str_out_off[u + 1]; ++e)
+        adj[n - 1 - ds->str_out_dst[e]].push_back(uint32_t(n - 1 - u));
+    r->str_out_off.assign(n + 1, 0);
+    for (int64_t t = 0; t < n; ++t) {
+      std::sort(adj[t].begin(), adj[t].end());
+      r->str_out_dst.insert(r->str_out_dst.end(), adj[t].begin(), adj[t].end());
+      r->str_out_off[t + 1] = r->str_out_dst.size();
+    }
+    expand_char_graph(*r);
+    if (r->in_src.size() != ds->in_src.size())
+      throw std::runtime_error("the char graph is not the expansion of the string graph: cannot reverse it");
+    build_ch(*r, seed);
+    validate_dataset(*r);
+    *out = r.release();
+  })
+}
+
 int chsga_host_write_csr(const chsga_dataset* ds, const char* string_csr, const char* char_csr) {
   if (!ds || !string_csr || !char_csr) {
     set_error("null argument");

@@ -102,6 +102,29 @@ class DatasetInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PasgalParams(C.Structure):
+    _fields_ = [("match", C.c_int32), ("mismatch", C.c_int32), ("ins", C.c_int32), ("dele", C.c_int32),
+                ("score_only", C.c_int32)]
+
+
+class PasgalResult(C.Structure):
+    _fields_ = [
+        ("score", C.c_int32), ("strand", C.c_char), ("flags", C.c_int32),
+        ("qry_start", C.c_int64), ("qry_end", C.c_int64), ("ref_start", C.c_int64), ("ref_end", C.c_int64),
+        ("score_fwd", C.c_int32), ("score_rc", C.c_int32),
+        ("qry_end_fwd", C.c_int64), ("ref_end_fwd", C.c_int64), ("qry_end_rc", C.c_int64), ("ref_end_rc", C.c_int64),
+        ("rev_best", C.c_int32), ("cigar", C.c_void_p),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "cigar"}
+        d["strand"] = self.strand.decode() if self.strand else ""
+        d["cigar"] = C.string_at(self.cigar).decode() if self.cigar else None
+        return d
+
+
+PASGAL_EMPTY_WINDOW, PASGAL_PIN_MOVED = 1, 2
+
 ENGINE_SYMBOLS = {
     "chsga_last_error": (C.c_char_p, []),
     "chsga_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -130,6 +153,9 @@ ENGINE_SYMBOLS = {
     "chsga_traceback_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), i64p, C.c_int64, C.c_int64,
                                         C.POINTER(BatchOpts), C.c_int64, C.POINTER(C.POINTER(Path_)),
                                         C.POINTER(Stats)]),
+    "chsga_pasgal_align": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PasgalParams), C.POINTER(C.c_char_p), i64p,
+                                     C.c_int64, C.POINTER(PasgalResult), C.POINTER(Stats)]),
+    "chsga_pasgal_free": (None, [C.POINTER(PasgalResult), C.c_int64]),
     "chsga_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_int,
                                    C.POINTER(C.c_void_p)]),
     "chsga_mlp_destroy": (None, [C.c_void_p]),
@@ -156,6 +182,7 @@ HOST_SYMBOLS = {
     "chsga_host_save_ch": (C.c_int, [C.c_void_p, C.c_char_p]),
     "chsga_host_build_ch_file": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint64]),
     "chsga_host_describe": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(GraphDesc)]),
+    "chsga_host_reverse": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "chsga_host_info": (C.c_int, [C.c_void_p, C.POINTER(DatasetInfo)]),
     "chsga_host_free": (None, [C.c_void_p]),
     "chsga_host_gen_reads": (C.c_int, [C.c_void_p, C.POINTER(ReadParams), C.POINTER(C.c_void_p)]),

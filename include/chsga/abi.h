@@ -16,6 +16,8 @@
  *                            counterpart: the reference aligns reads one by one,
  *                            src/align.hpp:488)
  *   chsga_ctx_destroy     <- the delete[] of the row buffers (src/align.hpp:526)
+ *   chsga_pasgal_align    <- PaSGAL_match's alignToDAGLocal, the reference's
+ *                            baseline (src/PaSGALoverrides/align.hpp:500-700)
  *
  * Rules of the boundary:
  *  - plain pointers and sizes only; no exceptions cross it; every function
@@ -302,6 +304,61 @@ void chsga_path_free(chsga_path* p);
  * Replaces align_combo_with_traceback (src/traceback.hpp:1040-1053). */
 int chsga_traceback_batch(chsga_ctx* ctx, const char* const* reads, const int64_t* lens, int64_t n, int64_t k,
                           const chsga_batch_opts* opts, int64_t budget_bytes, chsga_path** paths, chsga_stats* stats);
+
+/*
+ * PaSGAL_match, the reference's baseline aligner (Makefile:197-213 builds
+ * PaSGAL's scalar code with src/PaSGALoverrides/align.hpp in place of its
+ * own align.hpp): local alignment of each read to a char DAG, maximising
+ * match - mismatch - ins - del scores with a floor of 0.
+ *
+ *   chsga_pasgal_align  <- alignToDAGLocal (src/PaSGALoverrides/align.hpp:500-700):
+ *                          phase 1 on the read and its reverse complement
+ *                          (alignToDAGLocal_Phase1_scalar, :43-132, N matches
+ *                          any char), the strand with the higher score ('-'
+ *                          on ties, :601-613), phase 1 in reverse for the
+ *                          start (alignToDAGLocal_Phase1_rev_scalar,
+ *                          :146-245, exact match) and phase 2, the DP
+ *                          recomputed in the [start, end] window and walked
+ *                          into a CIGAR (alignToDAGLocal_Phase2, :256-491,
+ *                          exact match, cigarCompact)
+ *   chsga_pasgal_free   <- the release of BestScoreInfo's cigar strings
+ *
+ * Each DP row is the plain path's row in negated scores, closed by the exact
+ * insertion closure; fwd and rev must be created with
+ * CHSGA_CREATE_EXACT_CLOSURE, fwd on the char DAG and rev on its reversal
+ * (char j becomes V - 1 - j; chsga_host_reverse builds it).  Char ids must
+ * be a topological order (every in-neighbour id smaller), as PaSGAL's loader
+ * numbers its CSR_char_container.  rev may be NULL when score_only.
+ */
+typedef struct chsga_pasgal_params {
+  int32_t match, mismatch, ins, del; /* PaSGAL's Parameters, each 1..63 (PaSGAL's defaults are 1; the
+                                        bound keeps its int8 vertical differences exact, align.hpp:347) */
+  int32_t score_only;                /* 1: phase 1 only, as PaSGAL built with PASGAL_SCORE_ONLY */
+} chsga_pasgal_params;
+
+/* result flags */
+#define CHSGA_PASGAL_EMPTY_WINDOW 1 /* phase 1-R put the start after the end: no phase 2 (the reference
+                                       would size a negative window) */
+#define CHSGA_PASGAL_PIN_MOVED 2    /* phase 1-R: the forward end cell, set to match + 1, was raised by the
+                                       closure, i.e. the reference's assertion currentMax == match
+                                       (align.hpp:221) fails; only where the forward pass's N-matches
+                                       disagree with the reverse pass's exact matches */
+
+typedef struct chsga_pasgal_result {
+  int32_t score;                 /* BestScoreInfo.score of the chosen strand */
+  char strand;                   /* '+' or '-' */
+  int32_t flags;
+  int64_t qry_start, qry_end;    /* qryRowStart, qryRowEnd: rows of the chosen strand's read (start -1 when score_only) */
+  int64_t ref_start, ref_end;    /* refColumnStart, refColumnEnd: char ids */
+  int32_t score_fwd, score_rc;   /* phase 1's best on each strand */
+  int64_t qry_end_fwd, ref_end_fwd, qry_end_rc, ref_end_rc;
+  int32_t rev_best;              /* phase 1-R's best (score + 1 when the assertion above holds) */
+  char* cigar;                   /* compact CIGAR of =, X, I, D (NULL when score_only); chsga_pasgal_free */
+} chsga_pasgal_result;
+
+int chsga_pasgal_align(chsga_ctx* fwd, chsga_ctx* rev, const chsga_pasgal_params* params, const char* const* reads,
+                       const int64_t* lens, int64_t n, chsga_pasgal_result* out, chsga_stats* stats);
+void chsga_pasgal_free(chsga_pasgal_result* res, int64_t n);
 
 /*
  * Learned cost model (no counterpart in the reference, which schedules

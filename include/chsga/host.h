@@ -17,6 +17,8 @@
  *   chsga_host_write_csr   <- handlegraph_to_csr (src/handlegraph_to_csr.cpp:39-150
  *                             and its char expansion, :165-247), reading GFA
  *                             in place of a libbdsg PackedGraph
+ *   chsga_host_reverse     <- the reversed traversal of PaSGAL's phase 1-R
+ *                             (src/PaSGALoverrides/align.hpp:181-197)
  *   chsga_host_generate,
  *   chsga_host_gen_reads   <- no counterpart (the reference ships no data)
  */
@@ -83,6 +85,12 @@ int chsga_host_save_ch(const chsga_dataset* ds, const char* path);
  * -v, src/CHoverrides/build_pch.cpp:96-114). */
 int chsga_host_build_ch_file(const char* str_adj, const char* label_offsets, const char* out_ch, uint64_t seed);
 int chsga_host_describe(const chsga_dataset* ds, int32_t alphabet, chsga_graph_desc* out);
+/* The reversed graph PaSGAL's phase 1-R aligns to (it walks chars V-1..0
+ * over out-edges, src/PaSGALoverrides/align.hpp:181-197): string s becomes
+ * n_s - 1 - s with its label reversed and every edge turned around, so char
+ * j becomes V - 1 - j; with its own CH.  The char graph must expand the
+ * string graph. */
+int chsga_host_reverse(const chsga_dataset* ds, uint64_t seed, chsga_dataset** out);
 int chsga_host_info(const chsga_dataset* ds, chsga_dataset_info* out);
 void chsga_host_free(chsga_dataset* ds);
 

@@ -19,6 +19,8 @@
  *                         over the full score matrix; oracle_validate_path is a
  *                         path checker stronger than ConSGA_traceback.cpp:184-221
  *                         (it also checks every step follows a char edge).
+ *   O5  oracle_pasgal_*   PaSGAL_match's local alignment, the reference's
+ *                         scalar baseline (src/PaSGALoverrides/align.hpp).
  */
 #ifndef CHSGA_ORACLE_H
 #define CHSGA_ORACLE_H
@@ -58,6 +60,15 @@ int64_t oracle_traceback(const chsga_graph_desc* g, const char* read, int64_t le
  *   4 vertex out of range, 5 last consuming row != len-1, 6 cost != score */
 int oracle_validate_path(const chsga_graph_desc* g, const char* read, int64_t len, const chsga_align_op* ops,
                          int64_t n_ops, int32_t score, int64_t* bad_index);
+
+/* O5: PaSGAL_match's alignToDAGLocal (src/PaSGALoverrides/align.hpp:500-700),
+ * scalar, one read per OpenMP thread: phase 1 on both strands, phase 1-R,
+ * phase 2 and the compact CIGAR (oracle/pasgal.cpp).  out[n]; free the
+ * CIGARs with oracle_pasgal_free.  CHSGA_EINVAL when the char ids are not a
+ * topological order or a read has a char outside ACGTN. */
+int oracle_pasgal_align(const chsga_graph_desc* g, const char* reads, const int64_t* off, int64_t n,
+                        const chsga_pasgal_params* params, chsga_pasgal_result* out);
+void oracle_pasgal_free(chsga_pasgal_result* res, int64_t n);
 
 /* sigma(a, b) for the given alphabet (4 = matchN, 5 = mismatchN). */
 int32_t oracle_sigma(int32_t alphabet, char a, char b, int32_t match, int32_t subst);

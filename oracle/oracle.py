@@ -35,6 +35,10 @@ def lib() -> C.CDLL:
         L.oracle_traceback.argtypes = [G, C.c_char_p, C.c_int64, C.POINTER(N.AlignOp), C.c_int64, N.i32p]
         L.oracle_validate_path.argtypes = [G, C.c_char_p, C.c_int64, C.POINTER(N.AlignOp), C.c_int64, C.c_int32,
                                            N.i64p]
+        L.oracle_pasgal_align.argtypes = [G, C.c_char_p, N.i64p, C.c_int64, C.POINTER(N.PasgalParams),
+                                          C.POINTER(N.PasgalResult)]
+        L.oracle_pasgal_free.restype = None
+        L.oracle_pasgal_free.argtypes = [C.POINTER(N.PasgalResult), C.c_int64]
         L.oracle_sigma.restype = C.c_int32
         L.oracle_sigma.argtypes = [C.c_int32, C.c_char, C.c_char, C.c_int32, C.c_int32]
         _LIB = L
@@ -119,3 +123,19 @@ def validate_path(desc: N.GraphDesc, read: bytes, ops: np.ndarray, score: int):
 
 def sigma(alphabet: int, a: bytes, b: bytes, match: int = 0, subst: int = 1) -> int:
     return lib().oracle_sigma(alphabet, a, b, match, subst)
+
+
+def pasgal_align(desc: N.GraphDesc, reads, match: int = 1, mismatch: int = 1, ins: int = 1, dele: int = 1,
+                 score_only: bool = False) -> list:
+    """O5: PaSGAL_match's alignToDAGLocal, one dict per read (as
+    chsga_amd.api.PasgalAligner.align)."""
+    n = len(reads)
+    off = np.ascontiguousarray(reads.offsets, np.int64)
+    prm = N.PasgalParams(match, mismatch, ins, dele, 1 if score_only else 0)
+    res = (N.PasgalResult * max(n, 1))()
+    _ok(lib().oracle_pasgal_align(C.byref(desc), reads.data, off.ctypes.data_as(N.i64p), n, C.byref(prm), res),
+        "pasgal")
+    try:
+        return [res[i].as_dict() for i in range(n)]
+    finally:
+        lib().oracle_pasgal_free(res, n)

@@ -230,13 +230,13 @@ def test_engine_rejects_bad_descriptor_without_touching_the_gpu():
 
 def test_drivers_under_every_reference_name():
     """The ConSGA targets of the reference's Makefile (:121-124, :142-145,
-    :156-171) are built; each rejects an unknown flag with its usage line and
+    :156-171) and PaSGAL_match (:197-213) are built; each rejects an unknown flag with its usage line and
     a missing input with an error, before any GPU call."""
     import subprocess
     bin_ = ROOT / "chsga_amd" / "bin"
     names = [f"ConSGA_{a}{s}" for a in ("matchN", "mismatchN") for s in ("", "_dag")] + \
         [f"ConSGA_traceback_{a}{s}" for a in ("matchN", "mismatchN") for s in ("", "_dag")] + \
-        ["ConSGA_traceback_mismatchN_dag_800gb"]
+        ["ConSGA_traceback_mismatchN_dag_800gb", "PaSGAL_match"]
     for n in names:
         assert (bin_ / n).exists(), n
         bad = subprocess.run([str(bin_ / n), "-a", "x"], capture_output=True, text=True, timeout=60)
