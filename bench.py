@@ -193,6 +193,44 @@ def cpu_baseline(ds, wl, reads, budget_s: float, gpu_engine) -> dict:
     }
 
 
+def pasgal_leg(ds, stream, args, no_cpu: bool) -> dict:
+    """PaSGAL_match (the reference's baseline local aligner) on the same
+    graph: GCUPS of phase 1 alone (both strands) and of the whole alignment
+    (phases 1, 1-R and 2 with CIGARs), each over the DP cells it computes,
+    on --pasgal-reads reads cut to --pasgal-len chars; and its CPU
+    restatement on a short prefix of read 0, whose results must equal the
+    GPU's."""
+    from chsga_amd.api import PasgalAligner
+    reads = type(stream).from_list([stream[i][:args.pasgal_len] for i in range(args.pasgal_reads)])
+    t = time.perf_counter()
+    al = PasgalAligner(ds)
+    setup = time.perf_counter() - t
+    out = {"reads": len(reads), "read_chars": args.pasgal_len, "context_setup_s": round(setup, 1)}
+    for mode, so in (("phase1", True), ("full", False)):
+        t = time.perf_counter()
+        res = al.align(reads, score_only=so)
+        dt = time.perf_counter() - t
+        out[f"{mode}_gcups"] = round(al.last_stats["cells"] / dt / 1e9, 3)
+        out[f"{mode}_s"] = round(dt, 2)
+    out["scores"] = [r["score"] for r in res]
+    if not no_cpu:
+        from oracle import oracle as O
+        sample = type(stream).from_list([stream[0][:48]])
+        t = time.perf_counter()
+        exp = O.pasgal_align(ds.desc(), sample)
+        dt = time.perf_counter() - t
+        V = ds.info()["V"]
+        e = exp[0]
+        window = max(0, e["qry_end"] - e["qry_start"] + 1) * max(0, e["ref_end"] - e["ref_start"] + 1)
+        cells = 3 * len(sample[0]) * V + window  # phases 1 (two strands) and 1-R, phase 2 on its window
+        got = al.align(sample)
+        out["cpu"] = {"gcups": round(cells / dt / 1e9, 4), "cores": 1, "kind": "port",
+                      "sample": f"read 0 cut to {len(sample[0])} chars, whole alignment, one thread",
+                      "exact": got == exp}
+    al.close()
+    return out
+
+
 def main() -> int:
     t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
@@ -221,6 +259,9 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run a torch copy of N MiB (PMC scaling)")
     ap.add_argument("--traceback", type=int, default=0,
                     help="also time the per-read checkpointed traceback of this many reads")
+    ap.add_argument("--pasgal-reads", type=int, default=2,
+                    help="reads (cut to --pasgal-len chars) of the PaSGAL_match comparison leg (0: none)")
+    ap.add_argument("--pasgal-len", type=int, default=1000)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -277,7 +318,7 @@ def main() -> int:
     # time reserved for the legs after the batched timed region: plain leg,
     # roofline profile, path checks, CPU baseline
     reserve = (args.plain_budget + 15.0 if args.mode in ("both", "plain") else 0.0) + 25.0 + \
-        (0.0 if args.no_cpu else args.cpu_budget + 15.0)
+        (0.0 if args.no_cpu else args.cpu_budget + 15.0) + (30.0 if args.pasgal_reads > 0 else 0.0)
 
     # ---- batched path -------------------------------------------------------
     # A step is one batch of batch_reads reads through the 64 slots: scores
@@ -451,6 +492,11 @@ def main() -> int:
         result["traceback_per_read"] = {"reads": len(rs), "k": args.k, "gcups": round(cells1 / dt1 / 1e9, 3)}
         log(f"[bench] per-read traceback: {cells1 / dt1 / 1e9:.2f} GCUPS ({len(rs)} reads)")
 
+    # ---- PaSGAL_match, the reference's baseline aligner, for comparison ----
+    if args.pasgal_reads > 0:
+        result["pasgal"] = pasgal_leg(ds, stream, args, args.no_cpu)
+        log(f"[bench] PaSGAL_match: {result['pasgal']}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds, wl, stream, args.cpu_budget, eng)
@@ -482,6 +528,8 @@ def main() -> int:
     }
     if "traceback_per_read" in result:
         line["traceback_per_read"] = result["traceback_per_read"]
+    if "pasgal" in result:
+        line["pasgal_match"] = result["pasgal"]
     print(json.dumps(line), flush=True)
     return 0
 

@@ -156,6 +156,35 @@ def test_c3_prefixes_against_oracle_and_batched_traceback(c3, pool):
 
 
 @pytest.mark.timeout(600)
+def test_c3_pasgal_against_oracle(c3, pool):
+    """PaSGAL_match (src/PaSGALoverrides/align.hpp) on the c3 graph: two reads
+    cut to 80 chars, the second reverse-complemented.  Score, strand, ends,
+    starts and CIGAR of each equal O5 (phase 1 on both strands, phase 1-R,
+    phase 2), which runs on the host meanwhile; a read whose pinned end cell
+    the closure raised (CHSGA_PASGAL_PIN_MOVED, possible with this workload's
+    N chars) is compared on its phase-1 results."""
+    from chsga_amd.api import PasgalAligner
+    comp = bytes.maketrans(b"ACGTN", b"TGCAN")
+    w, ds, rs = c3
+    reads = ReadSet.from_list([rs[0][:80], rs[1][:80].translate(comp)[::-1]])
+    fut = pool.submit(O.pasgal_align, ds.desc(w.alphabet), reads)
+    al = PasgalAligner(ds)
+    got = al.align(reads)
+    al.close()
+    exp = fut.result()
+    _say("c3 PaSGAL", [(g["score"], g["strand"], g["qry_start"], g["qry_end"], g["flags"]) for g in got])
+    phase1 = ("score", "strand", "qry_end", "ref_end", "score_fwd", "score_rc", "qry_end_fwd", "ref_end_fwd",
+              "qry_end_rc", "ref_end_rc")
+    for i, (g, e) in enumerate(zip(got, exp)):
+        if g["flags"] & N.PASGAL_PIN_MOVED:
+            assert b"N" in reads[i], i
+            assert all(g[k] == e[k] for k in phase1), (i, g, e)
+        else:
+            assert g == e, (i, g, e)
+    assert got[1]["strand"] == "-"
+
+
+@pytest.mark.timeout(600)
 def test_c3_full_length_batched_traceback_validates(c3):
     """The headline work at full length: 64 c3 reads (log-normal, median
     3 kbp), batched traceback with checkpoint rows every 500 in HBM; every
