@@ -215,7 +215,7 @@ def pasgal_leg(ds, stream, args, no_cpu: bool) -> dict:
     out["scores"] = [r["score"] for r in res]
     if not no_cpu:
         from oracle import oracle as O
-        sample = type(stream).from_list([stream[0][:48]])
+        sample = type(stream).from_list([stream[0][:24]])
         t = time.perf_counter()
         exp = O.pasgal_align(ds.desc(), sample)
         dt = time.perf_counter() - t
@@ -318,7 +318,7 @@ def main() -> int:
     # time reserved for the legs after the batched timed region: plain leg,
     # roofline profile, path checks, CPU baseline
     reserve = (args.plain_budget + 15.0 if args.mode in ("both", "plain") else 0.0) + 25.0 + \
-        (0.0 if args.no_cpu else args.cpu_budget + 15.0) + (30.0 if args.pasgal_reads > 0 else 0.0)
+        (0.0 if args.no_cpu else args.cpu_budget + 15.0) + (45.0 if args.pasgal_reads > 0 else 0.0)
 
     # ---- batched path -------------------------------------------------------
     # A step is one batch of batch_reads reads through the 64 slots: scores

@@ -164,6 +164,8 @@ struct chsga_ctx {
   int64_t backward_tail = 6;      // batched backward: tail read by read (0: never)
   int64_t forward_tail = 3;       // batched forward: tail on the plain path (0: never)
   bool compact = false;  // char graph is the expansion of the string graph
+  int topo_ids = -1;     // chsga_pasgal_align: char ids are a topological order (1), not (0), unknown (-1)
+  const chsga_ctx* pasgal_rev = nullptr;  // the context last verified as this one's reversal
   bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)
   int64_t launches = 0, rows = 0;
   std::vector<hipEvent_t> ev_pool;

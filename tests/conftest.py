@@ -23,3 +23,11 @@ def c1():
     w = WORKLOADS["c1"]
     ds = w.dataset()
     return w, ds, w.readset(ds)
+
+
+@pytest.fixture(scope="session")
+def c5_ds():
+    """The c5 graph (100 Mbp backbone, ~117M chars), generated once per session."""
+    from chsga_amd import WORKLOADS
+    w = WORKLOADS["c5"]
+    return w, w.dataset()

@@ -226,13 +226,12 @@ def test_c3_full_size_properties():
     eng.close()
 
 
-def test_c5_large_graph_matchN():
+def test_c5_large_graph_matchN(c5_ds):
     """The largest workload's graph (116.7M chars, 28 CH levels, structural
     bubbles, matchN): row offsets exceed 2^31 elements, so every index is
     64-bit.  Batched and plain scores agree on short reads; every score lies in
     [0, |r|]; error-free reads along graph paths score 0."""
-    w = WORKLOADS["c5"]
-    ds = w.dataset()
+    w, ds = c5_ds
     rs = w.readset(ds, n_reads=10)
     reads = ReadSet.from_list([rs[i][:1500] for i in range(10)])
     clean = ReadSet.generate(ds, seed=7, n_reads=2, length=500)

@@ -257,14 +257,13 @@ def test_checkpoints_in_host_memory_and_two_level_segments(k):
     eng.close()
 
 
-def test_c5_traceback_validates():
+def test_c5_traceback_validates(c5_ds):
     """Traceback on the largest workload's graph (116.7M chars, structural
     bubbles, matchN, k = 500): the batched traceback falls back to the
     read-by-read backward (64-slot row buffers do not fit there); checkpoint
     rows in pinned host memory with the two-level segment store give the same
     path; every path validates and rescores to the batched score."""
-    w = WORKLOADS["c5"]
-    ds = w.dataset()
+    w, ds = c5_ds
     rs = w.readset(ds, n_reads=4)
     clean = ReadSet.generate(ds, seed=9, n_reads=1, length=300)
     reads = ReadSet.from_list([rs[0][:600], rs[1][:400], clean[0]])
