@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <exception>
@@ -869,6 +870,8 @@ int chsga_ctx_create_ex(const chsga_graph_desc* g, int device, uint32_t flags, c
     if (flags & ~uint32_t(CHSGA_CREATE_CHAR_ROWS | CHSGA_CREATE_EXACT_CLOSURE | CHSGA_CREATE_CUT_CLOSURE))
       throw std::invalid_argument("unknown context flags");
     c = new chsga_ctx();
+    static std::atomic<uint64_t> next_serial{1};
+    c->serial = next_serial++;
     try {
       c->device = device;
       c->allow_packed = (flags & CHSGA_CREATE_CHAR_ROWS) == 0;

@@ -165,7 +165,8 @@ struct chsga_ctx {
   int64_t forward_tail = 3;       // batched forward: tail on the plain path (0: never)
   bool compact = false;  // char graph is the expansion of the string graph
   int topo_ids = -1;     // chsga_pasgal_align: char ids are a topological order (1), not (0), unknown (-1)
-  const chsga_ctx* pasgal_rev = nullptr;  // the context last verified as this one's reversal
+  uint64_t serial = 0;   // unique per context of the process
+  uint64_t pasgal_rev = 0;  // serial of the context last verified as this one's reversal (0: none)
   bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)
   int64_t launches = 0, rows = 0;
   std::vector<hipEvent_t> ev_pool;
