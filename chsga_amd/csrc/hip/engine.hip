@@ -945,6 +945,10 @@ int chsga_ctx_set_option(chsga_ctx* ctx, int option, int64_t value) {
         if (value < 0) throw std::invalid_argument("CHSGA_OPT_BACKWARD_TAIL must be >= 0");
         ctx->backward_tail = value;
         break;
+      case CHSGA_OPT_BACKWARD_BAND:
+        if (value < 0) throw std::invalid_argument("CHSGA_OPT_BACKWARD_BAND must be >= 0");
+        ctx->backward_band = value;
+        break;
       case CHSGA_OPT_CUT_CLOSURE:
         if (value != 0 && !ctx->cut)
           throw std::invalid_argument("no cut tables: create the context with CHSGA_CREATE_CUT_CLOSURE");

@@ -162,9 +162,10 @@ struct chsga_ctx {
   int64_t seg_rows_cap = -1;      // per-read traceback: segment store rows (-1: by free memory)
   int64_t ring_rows_cap = -1;     // batched backward: ring rows at most (-1: by free memory)
   int64_t backward_tail = 6;      // batched backward: tail read by read (0: never)
+  int64_t backward_band = 1;      // batched backward: banded rounds of that many chars (1: the default width; 0: off)
   int64_t forward_tail = 3;       // batched forward: tail on the plain path (0: never)
   bool compact = false;  // char graph is the expansion of the string graph
-  int topo_ids = -1;     // chsga_pasgal_align: char ids are a topological order (1), not (0), unknown (-1)
+  int topo_ids = -1;     // char ids are a topological order (1), not (0), unknown (-1)
   uint64_t serial = 0;   // unique per context of the process
   uint64_t pasgal_rev = 0;  // serial of the context last verified as this one's reversal (0: none)
   bool allow_narrow = true, narrow = false;  // int16 row storage (exact, see fits_narrow)

@@ -20,7 +20,7 @@ ALPHABET_MISMATCH_N = 5
 OP_DELETE, OP_MATCHSUB, OP_INSERT = 0, 1, 2
 OPT_PROFILE, OPT_USE_GRAPHS, OPT_STAGED_WALK, OPT_NARROW, OPT_BATCHED_BACKWARD = 1, 2, 5, 6, 7
 OPT_CKPT_HBM_ROWS, OPT_SEG_ROWS, OPT_PACKED_ROWS, OPT_TOP_CLOSURE, OPT_RING_ROWS = 8, 9, 10, 11, 12
-OPT_BACKWARD_TAIL, OPT_FORWARD_TAIL, OPT_CUT_CLOSURE = 13, 14, 15
+OPT_BACKWARD_TAIL, OPT_FORWARD_TAIL, OPT_CUT_CLOSURE, OPT_BACKWARD_BAND = 13, 14, 15, 16
 CREATE_CHAR_ROWS, CREATE_EXACT_CLOSURE, CREATE_CUT_CLOSURE = 1, 2, 4
 
 u64p = C.POINTER(C.c_uint64)

@@ -157,18 +157,22 @@ def test_c1_batched_traceback(c1):
     eng.close()
 
 
-@pytest.mark.parametrize("ring", [-1, 0, 30])
+@pytest.mark.parametrize("band,ring", [(1, -1), (24, -1), (0, -1), (0, 0), (0, 30)])
 @pytest.mark.parametrize("k", [3, 13, 64])
-def test_batched_backward_equals_per_read_backward(k, ring):
-    """Lock-step segment recompute (up to 64 reads a round, snapshots every
-    k2 rows, a ring of k2 rows, all lanes walked at once) gives the paths of
-    recomputing each read's segments alone: more reads than slots, uneven
-    lengths (1..200 chars), int16 and int32 rows.  Ring plans: by free
-    memory (-1: here one ring of k rows), the smallest (0: k2 =
+def test_batched_backward_equals_per_read_backward(k, band, ring):
+    """Lock-step segment recompute (up to 64 reads a round, all lanes walked
+    at once) gives the paths of recomputing each read's segments alone: more
+    reads than slots, uneven lengths (1..200 chars), int16 and int32 rows.
+    Banded rounds (band > 0: one recompute, each lane's rows kept as a window
+    of chars below its walk; 1: the default 8192 chars, here the whole
+    graph's width; 24: so narrow that walks leave it and their segments are
+    redone on the plain path) and whole-row rounds (band 0) with ring plans by
+    free memory (-1: here one ring of k rows), the smallest (0: k2 =
     ceil(sqrt(k))) and at most 30 rows (k = 64: three snapshots, k2 = 22)."""
     ds, rs = small_workload(seed=45, n_reads=150, length=200, error=0.08, n_rate=0.02)
     rs = ReadSet.from_list([rs[i][: 1 + (i * 37) % 200] for i in range(len(rs))])
     eng = Engine(ds)
+    eng.set_option(N.OPT_BACKWARD_BAND, band)
     eng.set_option(N.OPT_RING_ROWS, ring)
     print("batched backward", k, ring, flush=True)
     a = eng.traceback_batch(rs, k=k)
