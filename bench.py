@@ -243,7 +243,8 @@ def main() -> int:
     ap.add_argument("--score-only", action="store_true", help="batched steps compute scores only (no paths)")
     ap.add_argument("--plain-reads", type=int, default=2, help="reads per plain step")
     ap.add_argument("--plain-budget", type=float, default=40.0, help="seconds for the plain leg's timed steps")
-    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=128,
+                    help="batched slots: 64 per slot group, each group on its own stream with its own backward worker")
     ap.add_argument("--mode", choices=["both", "plain", "batch"], default="both")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -409,12 +410,15 @@ def main() -> int:
         log(f"[bench] plain: {plain_gcups:.2f} GCUPS ({dt:.2f}s, {plain_steps} steps; {elapsed():.0f}s elapsed)")
 
     # ---- roofline of the dominant kernel (profiled, separate short run) ----
-    # HIP events bracket every row-kernel launch on the engine's own stream.
+    # HIP events bracket every row-kernel launch on the engine's own stream;
+    # one slot group (64 slots), so that no other group's kernels overlap the
+    # launches timed
     score_bytes = 2 if max(stream.lengths()) < 32767 else 4  # int16 storage when provably exact
     eng.set_option(N.OPT_PROFILE, 1)
-    prof_b = stream.subset(range(min(n_stream, 3 * args.slots)))
+    prof_slots = 64
+    prof_b = stream.subset(range(min(n_stream, 3 * prof_slots)))
     prof_b = type(prof_b).from_list([prof_b[i][:400] for i in range(len(prof_b))])
-    eng.score_batch(prof_b, slots=args.slots)
+    eng.score_batch(prof_b, slots=prof_slots)
     stb = eng.last_stats
     eng.align_combo(stream.subset([0]))
     stp = eng.last_stats
@@ -423,8 +427,8 @@ def main() -> int:
     p_ms = stp["row_kernel_ms"] / max(1, stp["row_kernel_launches"])
     packed = eng.packed
     kernel = "k_brow_packed" if packed else "k_brow_tile"
-    kb = (packed_kernel_bytes(ds, eng.schedule()["tile_chars"], score_bytes, args.slots) if packed
-          else batch_kernel_bytes(info, score_bytes, args.slots))
+    kb = (packed_kernel_bytes(ds, eng.schedule()["tile_chars"], score_bytes, prof_slots) if packed
+          else batch_kernel_bytes(info, score_bytes, prof_slots))
     pb = row_kernel_bytes(info, score_bytes)
     achieved = kb / (b_ms * 1e-3) / 1e9
     prof = profile_summary(kernel, kb)
@@ -441,7 +445,7 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("traffic_bytes_per_launch"),
                 "valu": valu, "salu": salu, "kernel": kernel, "kernel_avg_us": round(b_ms * 1e3, 2),
                 "bytes_per_launch": kb, "launches_timed": stb["row_kernel_launches"],
-                "score_storage_bytes": score_bytes,
+                "score_storage_bytes": score_bytes, "slots_profiled": prof_slots,
                 "plain_kernel": {"kernel": "k_row_tile8", "avg_us": round(p_ms * 1e3, 2), "bytes_per_launch": pb,
                                  "achieved_GBps": round(pb / (p_ms * 1e-3) / 1e9, 1),
                                  "sweep_post_avg_us_per_row": round(stp["sweep_ms"] * 1e3 /
