@@ -432,6 +432,7 @@ static void build_context(chsga_ctx* c, const chsga_graph_desc* g) {
   upload(c->d_tile_str, tile_str, s);
   upload(c->d_str_first, c->h_str_off, s);
   upload(c->d_sw_vert, sw_vert, s);
+  c->h_sw_vert = sw_vert;
   upload(c->d_lvl_off, c->h_lvl_off, s);
   {
     // the batched level sweep reads up to kSweepPad offsets past its level

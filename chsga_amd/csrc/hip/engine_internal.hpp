@@ -107,6 +107,10 @@ struct chsga_ctx {
   int64_t batch_big_level = 16384;
   int32_t batch_big_vpw = 8;
   std::vector<uint32_t> h_lvl_off, h_up_off, h_dn_off;
+  // string of each sweep position (levels in turn, each in ascending string
+  // id, so on a graph whose char ids are topological the strings below a
+  // char form a prefix of every level)
+  std::vector<uint32_t> h_sw_vert;
   // host copies for traceback
   std::string h_labels;
   std::vector<uint32_t> h_str_off, h_in_off, h_in_src, h_sin_off, h_sin_src;
