@@ -238,7 +238,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--batch-reads", type=int, default=512, help="reads per batched step")
+    ap.add_argument("--batch-reads", type=int, default=1024, help="reads per batched step")
     ap.add_argument("--k", type=int, default=500, help="checkpoint interval of the traceback")
     ap.add_argument("--score-only", action="store_true", help="batched steps compute scores only (no paths)")
     ap.add_argument("--plain-reads", type=int, default=2, help="reads per plain step")
