@@ -160,11 +160,12 @@ typedef struct chsga_stats {
                                      with 256 ticks or more left, those reads finish one by one on the plain
                                      path (same scores, argmins and checkpoint rows; default 3; 0: never) */
 #define CHSGA_OPT_BACKWARD_BAND 16 /* chsga_traceback_batch, batched backward: 1 (default) each round
-                                     recomputes its segments once and keeps each lane's rows as a window
-                                     of 8192 chars below the walk's position (> 1: a window of that many
-                                     chars), walking there and redoing a segment on the plain path where
-                                     the walk leaves the window; 0: whole rows (snapshots and a ring, each
-                                     row recomputed about twice).  Paths are the same in every case */
+                                     recomputes its segments once, its ticks stopping at the highest char
+                                     its walks start from, and keeps each lane's rows as a window of 8192
+                                     chars below the walk's position (> 1: a window of that many chars),
+                                     walking there and redoing a segment on the plain path where the walk
+                                     leaves the window; 0: whole rows (snapshots and a ring, each row
+                                     recomputed about twice).  Paths are the same in every case */
 #define CHSGA_OPT_CUT_CLOSURE 15   /* 1 (default on contexts created with CHSGA_CREATE_CUT_CLOSURE): the
                                      plain path closes rows on the match-cut components; 0: with the CH
                                      sweep (same values).  1 on other contexts: CHSGA_EINVAL */
