@@ -9,7 +9,9 @@ on the plain path (rows - 1 DP rows) and all of them on the batched path
 (`rows` ticks of `slots` slots, in groups of 64), printing one line per stage.
 `--mode traceback` instead runs the batched traceback of those reads with
 checkpoint rows every `k` rows: every lane stores one on the same ticks
-(rows 0, k, 2k, ...), the most a capture tick can hold.  `--closure` picks
+(rows 0, k, 2k, ...), the most a capture tick can hold; with more reads than
+slots the backward's banded rounds each take 64 reads in graph order, so
+their ticks stop at increasing chars.  `--closure` picks
 the rows' insertion closure: ConSGA's (default), the exact closure by the CH
 sweep (CHSGA_CREATE_EXACT_CLOSURE) or by match-cut components on the plain
 path (CHSGA_CREATE_CUT_CLOSURE).
